@@ -1,0 +1,98 @@
+/*
+ * st_api.h -- C ABI of the MI355X-native first-arrival tomography path.
+ *
+ * Drop-in boundary for the eikonal and least-squares tomography path of
+ * phbastosa/seismic_tomography_3D.  Plain pointers and sizes only; every
+ * call returns an st_status (0 = ST_OK).  Host arrays are owned by the
+ * caller; the context owns all device memory and reuses it across calls.
+ * A context is not thread-safe; calls are synchronous at return.
+ *
+ * Volumes are FP32, z fastest: idx = z + x*nz + y*nx*nz (the reference's
+ * layout, src/eikonal/block_FIM/block_FIM.cpp:17).  Point triples
+ * (sources, receivers) are interleaved x, y, z in metres.
+ */
+#ifndef ST_API_H
+#define ST_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ST_OK = 0,
+    ST_ERR_ARG = 1,          /* bad argument / size / pointer             */
+    ST_ERR_HIP = 2,          /* HIP runtime error (message in st_last_error) */
+    ST_ERR_NO_MODEL = 3,     /* st_set_model not called                   */
+    ST_ERR_DIVERGED = 4,     /* ray exceeded its step cap or lost its gradient */
+    ST_ERR_NOT_UPPER_BOUND = 5, /* warm start violated the upper-bound check */
+    ST_ERR_CAPACITY = 6,     /* an output buffer is too small             */
+    ST_ERR_NO_SURROGATE = 7, /* warm start requested without a network    */
+    ST_ERR_STATE = 8         /* call order (e.g. cgls before forward)     */
+} st_status;
+
+typedef struct st_ctx st_ctx;
+
+/* Eikonal solver options (block Fast Iterative Method). */
+typedef struct {
+    int block;          /* block edge: 4 or 8 (0 = tuned default)             */
+    int batch;          /* shots per persistent solve; 1 = one shot at a time  */
+    int max_inner;      /* inner iteration cap per block visit (0 = default)   */
+    float eps;          /* change threshold in seconds (<0 = default 0)        */
+    int warm;           /* 0 cold start, 1 surrogate warm start                */
+    float delta;        /* warm start inflation (1+delta) (0 = 0.02)           */
+    int judge;          /* 1: verify warm-started shots, re-solve cold on failure */
+    int grid;           /* workgroups per round launch (0 = auto)              */
+    int rounds_per_sync;/* rounds enqueued between host checks (0 = auto)      */
+    int use_graph;      /* 1: replay rounds from a captured hipGraph           */
+} st_fim_opts;
+
+typedef struct {
+    long long rounds;       /* FIM rounds executed (summed over batches)  */
+    long long visits;       /* block visits                               */
+    long long inner_iters;  /* inner iterations summed over visits        */
+    long long fallbacks;    /* warm-started shots re-solved cold          */
+    long long batches;      /* persistent solves launched                 */
+    double solve_ms;        /* device time of the FIM rounds              */
+    double total_ms;        /* wall time of the call                      */
+} st_fim_stats;
+
+/* Fills opts with defaults. */
+void st_fim_default_opts(st_fim_opts *opts);
+
+/* Context lifetime.  Replaces the per-shot cudaMalloc/cudaFree of
+ * block_FIM_solver (src/eikonal/block_FIM/cuda_kernel.cu:54-68,208-214). */
+int st_ctx_create(int device, st_ctx **out);
+int st_ctx_destroy(st_ctx *ctx);
+const char *st_last_error(const st_ctx *ctx);
+
+/* Slowness model (s/m), z fastest, nx*ny*nz.  Replaces
+ * Block_FIM::set_parameters' model read (block_FIM.cpp:85-91),
+ * prepare_volumes/expand_model (:142-188, :8-57) and apply_model_mask
+ * (:264-293): the padded, block-major copy lives on the device.  The FIM
+ * uses dx as the isotropic spacing, as the reference does (:173). */
+int st_set_model(st_ctx *ctx, const float *slowness, int nx, int ny, int nz,
+                 float dx, float dy, float dz);
+
+/* Current device slowness back to the host (z fastest, nx*ny*nz). */
+int st_get_model(st_ctx *ctx, float *slowness);
+
+/* Solve nshot point sources and pick nrcv receivers per shot.  Replaces
+ * the shot loop of eikonal_main.cpp:31-39 over
+ * Block_FIM::eikonal_equation (block_FIM.cpp:212-220: apply_source_time,
+ * block_FIM_solver, extract_solution) and write_first_arrival (:228-262).
+ * picks: nshot*nrcv (shot-major).  T_out (nullable): nshot*nx*ny*nz.
+ * opts/stats nullable. */
+int st_fim_solve(st_ctx *ctx, const float *src_xyz, int nshot,
+                 const float *rcv_xyz, int nrcv, float *picks, float *T_out,
+                 const st_fim_opts *opts, st_fim_stats *stats);
+
+/* Receiver picks on a caller volume: write_first_arrival's trilinear
+ * (block_FIM.cpp:228-262, trilinear.cpp:3-20) on the device. */
+int st_picks_volume(st_ctx *ctx, const float *T, int nx, int ny, int nz,
+                    float dx, float dy, float dz, const float *rcv_xyz,
+                    int nrcv, float *picks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,124 @@
+// st_common.hpp -- shared definitions of the HIP implementation behind
+// include/st_api.h.  gfx950 (CDNA4) only: wave64, one wave per block visit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "st_api.h"
+
+namespace st {
+
+constexpr float kInf = 1e6f;  // "not reached" (cuda_kernel.cuh:7)
+
+// Thrown inside the library, turned into a status at the C boundary.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define ST_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            throw ::st::Error{ST_ERR_HIP, std::string(#call) + ": " +         \
+                                              hipGetErrorString(e_)};         \
+    } while (0)
+
+#define ST_REQUIRE(cond, code, text)                                          \
+    do {                                                                      \
+        if (!(cond)) throw ::st::Error{(code), (text)};                       \
+    } while (0)
+
+// Growable device buffer owned by the context (allocated once per size,
+// reused across shots and iterations).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) ST_HIP(hipFree(ptr));
+        ptr = nullptr;
+        ST_HIP(hipMalloc(&ptr, n));
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Block-major volume geometry.  The FIM domain is the reference's
+// pad-to-4 extent (n4*); blocks of edge B tile the pad-to-B extent, cells
+// beyond n4* are outside the domain and stay at kInf.
+struct BlockGeo {
+    int nx, ny, nz;        // model extent
+    int n4x, n4y, n4z;     // FIM domain (pad to 4, block_FIM.cpp:144-150)
+    int B;                 // block edge
+    int nbx, nby, nbz;     // blocks per axis
+    long long nblk;        // blocks
+    long long vol;         // cells per shot in block layout = nblk * B^3
+};
+
+inline BlockGeo make_geo(int nx, int ny, int nz, int B) {
+    auto pad = [](int n, int b) { return n + (b - n % b) % b; };
+    BlockGeo g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.n4x = pad(nx, 4); g.n4y = pad(ny, 4); g.n4z = pad(nz, 4);
+    g.B = B;
+    g.nbx = pad(g.n4x, B) / B; g.nby = pad(g.n4y, B) / B; g.nbz = pad(g.n4z, B) / B;
+    g.nblk = (long long)g.nbx * g.nby * g.nbz;
+    g.vol = g.nblk * B * B * B;
+    return g;
+}
+
+// Address of cell (x, y, z) in block layout: block b = bx + nbx*(by +
+// nby*bz) (x fastest, as block_FIM.cpp:271-292), in-block t = tx + B*ty +
+// B*B*tz.
+__host__ __device__ inline long long blk_addr(int x, int y, int z, int B, int nbx, int nby) {
+    int bx = x / B, by = y / B, bz = z / B;
+    int tx = x - bx * B, ty = y - by * B, tz = z - bz * B;
+    long long b = bx + (long long)nbx * (by + (long long)nby * bz);
+    return b * (B * B * B) + tx + B * (ty + B * tz);
+}
+
+}  // namespace st
+
+// The context.
+struct st_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    // model
+    bool has_model = false;
+    int nx = 0, ny = 0, nz = 0;
+    float dx = 0, dy = 0, dz = 0;
+    st::DevBuf slow_zxy;               // slowness, z fastest (for updates)
+    st::DevBuf slow_blk[2];            // block layouts for B = 4, 8
+    bool blk_valid[2] = {false, false};
+
+    // FIM state
+    st::DevBuf T;                      // batch x vol travel times
+    st::DevBuf flags;                  // batch x nblk schedule bits
+    st::DevBuf list[2];                // active lists (by round parity)
+    st::DevBuf counters;               // list counts, stats
+    st::DevBuf src, rcv, picks, scratch;
+    void *host_pinned = nullptr;       // small pinned staging area
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    // graph cache: captured rounds for one (geometry, batch) shape
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    long long graph_key = -1;
+
+    // opaque extension state (tomography, surrogate)
+    void *tomo = nullptr;
+    void *surr = nullptr;
+};

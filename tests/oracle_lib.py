@@ -1,0 +1,167 @@
+"""ctypes access to the checker: oracle/liboracle.so (C restatement) and,
+when built, oracle/_ref/libstref.so (the reference's own CPU code)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libstref.so")
+
+fp = ctypes.POINTER(ctypes.c_float)
+dp = ctypes.POINTER(ctypes.c_double)
+ip = ctypes.POINTER(ctypes.c_int)
+lp = ctypes.POINTER(ctypes.c_long)
+I, F, D, L = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_long
+
+_O = None
+_R = None
+
+
+def P(a):
+    if a is None:
+        return None
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(fp)
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(ip)
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(lp)
+    raise TypeError(a.dtype)
+
+
+def oracle():
+    global _O
+    if _O is None:
+        o = ctypes.CDLL(ORACLE)
+        sig = {
+            "or_pad_extent": (I, [I, I]),
+            "or_pad_replicate": (None, [fp, I, I, I, fp, I, I, I]),
+            "or_godunov_f32": (F, [F, F, F, F, F]),
+            "or_godunov_f64": (D, [D, D, D, D, D]),
+            "or_source_seed": (None, [fp, I, I, I, F, F, F, F, F, F, lp, fp]),
+            "or_eikonal_fp64": (I, [fp, I, I, I, D, L, D, D, I, dp]),
+            "or_block_fim_ref": (I, [fp, I, I, I, F, L, F, I, fp, lp]),
+            "or_crop_f32": (None, [fp, I, I, I, fp, I, I, I]),
+            "or_crop_f64": (None, [dp, I, I, I, dp, I, I, I]),
+            "or_trilinear": (F, [F] * 17),
+            "or_picks": (None, [fp, I, I, I, F, F, F, fp, fp, fp, I, fp]),
+            "or_fsm": (None, [fp, I, I, I, F, F, F, F, F, F, fp]),
+            "or_fsm_batch": (None, [fp, I, I, I, F, F, F, fp, I, fp, fp, fp, I, fp, I]),
+            "or_expand_fdm": (None, [fp, I, I, I, fp]),
+            "or_ray_trace_shot": (L, [fp, I, I, I, F, F, F, I, I, I, F, F, F, F, F, F,
+                                      fp, fp, fp, I, I, ip, ip, fp, L, fp]),
+            "or_tk_stencil": (None, [I, ip]),
+            "or_cgls_f64": (None, [lp, ip, fp, I, I, fp, I, dp]),
+            "or_cgls_ref_f32": (None, [lp, ip, fp, I, I, fp, I, fp]),
+            "or_residual_norm": (F, [fp, fp, I]),
+            "or_gaussian": (None, [fp, fp, I, I, I, I, F]),
+            "or_gaussian_kernel": (None, [I, F, fp]),
+            "or_model_update": (None, [fp, fp, I, I, I, F, F, F, I, I, I, F, F, F, I, I, F, fp, fp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(o, name)
+            f.restype = res
+            f.argtypes = args
+        _O = o
+    return _O
+
+
+def ref_available():
+    return os.path.exists(REF)
+
+
+def ref():
+    global _R
+    if _R is None:
+        r = ctypes.CDLL(REF)
+        r.ref_eikonal.argtypes = [I, fp, I, I, I, F, F, F, fp, I, fp, fp, fp, I, fp, fp,
+                                  ctypes.c_char_p]
+        r.ref_eikonal.restype = I
+        r.ref_fsm_threads.argtypes = [fp, I, I, I, F, F, F, fp, I, fp, fp, fp, I, fp, I]
+        r.ref_fsm_threads.restype = I
+        r.ref_run_params.argtypes = [ctypes.c_char_p, I, ip, ip]
+        r.ref_run_params.restype = I
+        r.ref_trilinear.argtypes = [F] * 17
+        r.ref_trilinear.restype = F
+        r.ref_gaussian.argtypes = [fp, fp, I, I, I, I, F]
+        r.ref_gaussian.restype = None
+        r.ref_catch_parameter.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, I]
+        r.ref_catch_parameter.restype = I
+        r.ref_regular.argtypes = [ctypes.c_char_p, ctypes.c_char_p, fp, fp, fp, I]
+        r.ref_regular.restype = I
+        _R = r
+    return _R
+
+
+# ---------------------------------------------------------------- helpers --
+
+def padded_model(vel, B=4):
+    """Slowness padded to the FIM domain (pad-to-4, clamp replication)."""
+    o = oracle()
+    s = np.ascontiguousarray((1.0 / vel).astype(np.float32))
+    ny, nx, nz = s.shape
+    nxx, nyy, nzz = (o.or_pad_extent(n, B) for n in (nx, ny, nz))
+    S = np.zeros((nyy, nxx, nzz), np.float32)
+    o.or_pad_replicate(P(s), nx, ny, nz, P(S), nxx, nyy, nzz)
+    return s, S
+
+
+def eikonal_fp64(vel, h, src, tol=1e-12):
+    """FP64 fixed point of the block-FIM discretisation, cropped (ny,nx,nz)."""
+    o = oracle()
+    s, S = padded_model(vel)
+    ny, nx, nz = s.shape
+    nyy, nxx, nzz = S.shape
+    sid = ctypes.c_long()
+    t0 = ctypes.c_float()
+    o.or_source_seed(P(S), nxx, nyy, nzz, h, h, h, float(src[0]), float(src[1]), float(src[2]),
+                     ctypes.byref(sid), ctypes.byref(t0))
+    Tp = np.zeros(S.shape, np.float64)
+    npass = o.or_eikonal_fp64(P(S), nxx, nyy, nzz, h, sid.value, t0.value, tol, 1000, P(Tp))
+    assert npass > 0, "FP64 oracle did not converge"
+    T = np.zeros(s.shape, np.float64)
+    o.or_crop_f64(P(Tp), nxx, nyy, nzz, P(T), nx, ny, nz)
+    return T
+
+
+def block_fim_ref(vel, h, src, n_iter=100):
+    o = oracle()
+    s, S = padded_model(vel)
+    ny, nx, nz = s.shape
+    nyy, nxx, nzz = S.shape
+    sid = ctypes.c_long()
+    t0 = ctypes.c_float()
+    o.or_source_seed(P(S), nxx, nyy, nzz, h, h, h, float(src[0]), float(src[1]), float(src[2]),
+                     ctypes.byref(sid), ctypes.byref(t0))
+    Tp = np.zeros(S.shape, np.float32)
+    st = np.zeros(2, np.int64)
+    o.or_block_fim_ref(P(S), nxx, nyy, nzz, h, sid.value, t0.value, n_iter, P(Tp), P(st))
+    T = np.zeros(s.shape, np.float32)
+    o.or_crop_f32(P(Tp), nxx, nyy, nzz, P(T), nx, ny, nz)
+    return T, st
+
+
+def picks(T, spacing, rcv):
+    o = oracle()
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    ny, nx, nz = T.shape
+    rcv = np.ascontiguousarray(rcv, np.float32).reshape(-1, 3)
+    rx, ry, rz = (np.ascontiguousarray(rcv[:, i]) for i in range(3))
+    out = np.zeros(rcv.shape[0], np.float32)
+    o.or_picks(P(T), nx, ny, nz, *spacing, P(rx), P(ry), P(rz), rcv.shape[0], P(out))
+    return out
+
+
+def fsm(vel, spacing, src):
+    o = oracle()
+    s = np.ascontiguousarray((1.0 / vel).astype(np.float32))
+    ny, nx, nz = s.shape
+    T = np.zeros(s.shape, np.float32)
+    o.or_fsm(P(s), nx, ny, nz, *spacing, *(float(v) for v in src), P(T))
+    return T

@@ -1,0 +1,115 @@
+"""GPU parity of the block FIM (C ABI st_fim_solve) against the oracle.
+
+Tolerance: the metric's 1e-4 s on receiver traveltimes against the FP64
+fixed point of the same discretisation; full volumes are held to the same
+bound.  Sizes are ones the FP64 oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from seismic_tomography_3d_amd import synth
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def _case(nx=41, ny=37, nz=21, h=25.0, seed=1):
+    vel = synth.gradient_model(nx, ny, nz, h, seed=seed)
+    return vel, h
+
+
+@pytest.mark.parametrize("block", [4, 8])
+def test_fim_small_vs_fp64(ctx, block):
+    vel, h = _case()
+    ctx.set_model(1.0 / vel, h)
+    srcs = np.array([[500.0, 437.3, 110.0], [0.0, 0.0, 0.0], [1000.0, 900.0, 500.0],
+                     [12.5, 880.0, 260.0]], np.float32)
+    rcv = np.stack(synth.surface_grid(6, 25.0, 875.0, 0.0), axis=1)
+    picks, T, st = ctx.fim_solve(srcs, rcv, return_T=True, block=block, batch=4)
+    print(f"block={block} stats={st}")
+    for i, s in enumerate(srcs):
+        T64 = orc.eikonal_fp64(vel, h, s)
+        err = np.abs(T[i].astype(np.float64) - T64).max()
+        p64 = orc.picks(T64.astype(np.float32), (h, h, h), rcv)
+        perr = np.abs(picks[i].astype(np.float64) - p64).max()
+        print(f"shot {i} src={s} max|T-T64|={err:.3e} max|pick err|={perr:.3e}")
+        assert err < TOL
+        assert perr < TOL
+
+
+def test_single_shot_case_vs_fp64(ctx):
+    c = synth.single_shot_case()
+    ctx.set_model(1.0 / c["vel"], c["h"])
+    rcv = np.stack([c["rx"], c["ry"], c["rz"]], axis=1)
+    picks, T, st = ctx.fim_solve(c["src"], rcv, return_T=True, batch=1)
+    T64 = orc.eikonal_fp64(c["vel"], c["h"], c["src"][0])
+    err = np.abs(T[0] - T64).max()
+    p64 = orc.picks(T64.astype(np.float32), (c["h"],) * 3, rcv)
+    perr = np.abs(picks[0] - p64).max()
+    rms = np.sqrt(np.mean((picks[0] - p64) ** 2))
+    print(f"101x101x51: max|T-T64|={err:.3e} picks max={perr:.3e} rms={rms:.3e} stats={st}")
+    assert err < TOL and perr < TOL
+
+
+def test_batch_matches_single(ctx):
+    vel, h = _case(seed=3)
+    ctx.set_model(1.0 / vel, h)
+    srcs = synth.random_sources(9, 41, 37, h, 0.0, 500.0, seed=11)
+    rcv = np.stack(synth.surface_grid(5, 25.0, 875.0, 0.0), axis=1)
+    pb, Tb, _ = ctx.fim_solve(srcs, rcv, return_T=True, batch=9)
+    p1, T1, _ = ctx.fim_solve(srcs, rcv, return_T=True, batch=1)
+    assert np.abs(Tb - T1).max() < 1e-5
+    assert np.abs(pb - p1).max() < 1e-5
+
+
+def test_homogeneous_axis_exact(ctx):
+    """Along grid axes through the source the Godunov update is the 1D
+    branch: T = n*h*s exactly up to FP32 rounding."""
+    nx, ny, nz, h, v = 33, 29, 25, 10.0, 2000.0
+    vel = np.full((ny, nx, nz), v, np.float32)
+    ctx.set_model(1.0 / vel, h)
+    src = np.array([[160.0, 140.0, 120.0]], np.float32)
+    _, T, _ = ctx.fim_solve(src, None, return_T=True)
+    T = T[0]
+    n = np.arange(nx) - 16
+    expect = np.abs(n) * h / v
+    assert np.abs(T[14, :, 12] - expect).max() < 2e-6
+    m = np.arange(nz) - 12
+    assert np.abs(T[14, 16, :] - np.abs(m) * h / v).max() < 2e-6
+
+
+def test_layered_vertical_line(ctx):
+    """Flat layers: below the source T(k) = sum_{i<=k} h*s_i (1D branch)."""
+    nx, ny, nz, h = 21, 21, 40, 10.0
+    v = np.linspace(1500, 3500, nz).astype(np.float32)
+    vel = np.broadcast_to(v[None, None, :], (ny, nx, nz)).copy()
+    ctx.set_model(1.0 / vel, h)
+    src = np.array([[100.0, 100.0, 0.0]], np.float32)
+    _, T, _ = ctx.fim_solve(src, None, return_T=True)
+    s = (1.0 / v).astype(np.float64)
+    expect = np.concatenate([[0.0], np.cumsum(h * s[1:])])
+    assert np.abs(T[0][10, 10, :] - expect).max() < 5e-6
+
+
+def test_picks_volume_bit_exact(ctx):
+    rng = np.random.default_rng(5)
+    T = rng.uniform(0.0, 2.0, (13, 17, 11)).astype(np.float32)
+    sp = (12.5, 12.5, 12.5)
+    rcv = np.stack([rng.uniform(0, 16 * 12.5, 50), rng.uniform(0, 12 * 12.5, 50),
+                    rng.uniform(0, 10 * 12.5, 50)], axis=1).astype(np.float32)
+    # receivers exactly on the last nodes (the reference reads past the end)
+    rcv[:3] = [[16 * 12.5, 12 * 12.5, 10 * 12.5], [0, 0, 0], [16 * 12.5, 0, 5 * 12.5]]
+    gpu = ctx.picks_volume(T, sp, rcv)
+    cpu = orc.picks(T, sp, rcv)
+    assert np.array_equal(gpu, cpu)
+
+
+def test_rejects_source_outside(ctx):
+    from seismic_tomography_3d_amd import StError
+    vel, h = _case()
+    ctx.set_model(1.0 / vel, h)
+    with pytest.raises(StError):
+        ctx.fim_solve(np.array([[-1.0, 0.0, 0.0]], np.float32), None)
+    with pytest.raises(StError):
+        ctx.fim_solve(np.array([[0.0, 0.0, 1e5]], np.float32), None)
