@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Headline benchmark: eikonal shot-solves per second at 201x201x101.
+
+Workload: the batched case of seismic_tomography_3d_amd.synth -- 256 point
+sources (x, y uniform, depth U(0, 2 km), seed 4) on a 201x201x101 grid at
+h = 25 m, velocity rising linearly 1500 -> 4000 m/s with a -10% Gaussian
+anomaly (seed 1), 20x20 surface receivers.  One step = one pass of the hot
+path over the whole shot set: block-FIM solve of every shot plus receiver
+picks, with the model already resident in HBM.
+
+Legs reported in the one JSON line:
+  value           shot-solves/s of the headline method (batched persistent
+                  FIM, plus the learned warm start when a trained network is
+                  present: --warm auto)
+  plain_fim       the same solver one shot at a time, cold start
+  roofline        HBM roofline of the dominant kernel (the FIM round kernel):
+                  algorithmic bytes per block visit x visits / kernel time
+  cpu_baseline    the reference's own Accurate_FSM (oracle/_ref, built from the
+                  reference sources) on the host cores, bounded sample; this
+                  leg also runs the FP64 oracle on sample shots and reports the
+                  receiver-traveltime error of the GPU picks against it
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "shot-solves/sec @201x201x101, ≤1e-4 s err: batched+surrogate vs plain FIM"
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_visit(B):
+    """Algorithmic HBM bytes of one block visit: slowness B^3, travel times
+    B^3 + 6 B^2 (block + face halos) in and B^3 out, FP32."""
+    return 4 * (B ** 3 + (B ** 3 + 6 * B ** 2) + B ** 3)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_s=20.0):
+    """CPU reference timing + FP64 error check (the checker lives here)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+
+    vel, h = case["vel"], case["h"]
+    s = np.ascontiguousarray((1.0 / vel).astype(np.float32))
+    ny, nx, nz = s.shape
+    rcv = np.stack([case["rx"], case["ry"], case["rz"]], axis=1).astype(np.float32)
+    rx, ry, rz = (np.ascontiguousarray(rcv[:, i]) for i in range(3))
+    nr = rcv.shape[0]
+    src = np.ascontiguousarray(case["src"], np.float32)
+
+    # FP64 oracle on the sample shots, one thread per shot (ctypes frees the GIL)
+    errs = [None] * len(sample_idx)
+
+    def check(k, i):
+        T64 = orc.eikonal_fp64(vel, h, src[i])
+        p64 = orc.picks(T64.astype(np.float32), (h, h, h), rcv)
+        pe = gpu_picks[i].astype(np.float64) - p64
+        te = np.abs(gpu_T_sample[k].astype(np.float64) - T64).max()
+        errs[k] = (np.abs(pe).max(), float(np.sqrt(np.mean(pe ** 2))), te)
+
+    ths = [threading.Thread(target=check, args=(k, i)) for k, i in enumerate(sample_idx)]
+    t0 = time.time()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    log(f"fp64 check of {len(sample_idx)} shots took {time.time() - t0:.1f}s")
+    err = {
+        "shots_checked": [int(i) for i in sample_idx],
+        "max_pick_err_s": float(max(e[0] for e in errs)),
+        "rms_pick_err_s": float(np.sqrt(np.mean([e[1] ** 2 for e in errs]))),
+        "max_volume_err_s": float(max(e[2] for e in errs)),
+        "per_shot_max_pick_err_s": [float(e[0]) for e in errs],
+    }
+
+    # CPU reference: the reference's own Accurate_FSM when built, else the port
+    kind = "reference" if orc.ref_available() else "port"
+    n = threads
+    picks = np.zeros((n, nr), np.float32)
+    sub = np.ascontiguousarray(src[:n])
+    t0 = time.time()
+    if kind == "reference":
+        orc.ref().ref_fsm_threads(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), n, orc.P(rx),
+                                  orc.P(ry), orc.P(rz), nr, orc.P(picks), threads)
+    else:
+        orc.oracle().or_fsm_batch(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), n, orc.P(rx),
+                                  orc.P(ry), orc.P(rz), nr, orc.P(picks), threads)
+    dt = time.time() - t0
+    base = {
+        "value": n / dt,
+        "unit": "shot-solves/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{n} of the {len(src)} shots (one per thread), Accurate_FSM "
+                  f"(Noble-Gesret-Belayouni, the reference's default CPU eikonal) + picks, "
+                  f"201x201x101, {dt:.1f} s wall",
+    }
+    return base, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=256)
+    ap.add_argument("--block", type=int, default=0, help="4 or 8 (0 = tuned)")
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--max-inner", type=int, default=0)
+    ap.add_argument("--plain-shots", type=int, default=16)
+    ap.add_argument("--warm", default="auto", choices=["auto", "on", "off"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--check-shots", type=int, default=8)
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(shots shard across GPUs by design, see DESIGN.md)"}))
+        sys.exit(2)
+
+    from seismic_tomography_3d_amd import Context, synth, tune
+
+    case = synth.batched_case(a.shots)
+    h = case["h"]
+    src = case["src"]
+    rcv = np.stack([case["rx"], case["ry"], case["rz"]], axis=1).astype(np.float32)
+    ctx = Context(0)
+    ctx.set_model(1.0 / case["vel"], h)
+    log(f"model 201x201x101 resident; {len(src)} shots, {len(rcv)} receivers")
+
+    params = tune.best_params(ctx, src, rcv, block=a.block, grid=a.grid,
+                              max_inner=a.max_inner, log=log)
+    warm = 0
+    if a.warm in ("auto", "on"):
+        have = tune.load_surrogate(ctx, case, log=log)
+        if a.warm == "on" and not have:
+            raise SystemExit("--warm on but no trained network is available")
+        warm = 1 if have else 0
+    opts = dict(block=params["block"], grid=params["grid"], max_inner=params["max_inner"],
+                batch=a.batch, warm=warm)
+    log(f"options {opts}")
+
+    for _ in range(a.warmup):
+        ctx.fim_solve(src, rcv, **opts)
+    times, stats = [], None
+    for k in range(a.steps):
+        t0 = time.perf_counter()
+        picks, _, stats = ctx.fim_solve(src, rcv, **opts)
+        times.append(time.perf_counter() - t0)
+        log(f"step {k}: {times[-1] * 1e3:.1f} ms, {stats}")
+    ms = 1e3 * float(np.median(times))
+    value = len(src) / (ms / 1e3)
+
+    # plain FIM: same kernels, cold start, one shot at a time (bounded sample)
+    npl = min(a.plain_shots, len(src))
+    popts = dict(opts, batch=1, warm=0)
+    ctx.fim_solve(src[:2], rcv, **popts)
+    t0 = time.perf_counter()
+    _, _, pst = ctx.fim_solve(src[:npl], rcv, **popts)
+    pdt = time.perf_counter() - t0
+    plain = {"value": npl / pdt, "unit": "shot-solves/s", "shots": npl,
+             "ms_per_shot": 1e3 * pdt / npl, "rounds_per_shot": pst["rounds"] / npl,
+             "visits_per_shot": pst["visits"] / npl}
+    log(f"plain: {plain}")
+
+    # roofline of the FIM round kernel over the last timed step
+    B = params["block"]
+    bpv = bytes_per_visit(B)
+    achieved = stats["visits"] * bpv / (stats["solve_ms"] / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": f"fim_round{B}", "bytes_per_visit": bpv,
+            "visits_per_step": stats["visits"], "kernel_ms_per_step": stats["solve_ms"],
+            "launches_per_step": stats["rounds"]}
+    prof = os.path.join(ROOT, "profiles", f"fim_round{B}_batched_201x201x101_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        roof["traffic"] = tr.get("bytes_per_visit_measured")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "shot-solves/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched eikonal: 256 shots, 201x201x101, h=25 m, linear "
+                               "1500-4000 m/s + -10% anomaly, 400 surface receivers",
+                   "shots": len(src), "receivers": len(rcv), "block": B,
+                   "batch": a.batch, "grid": params["grid"],
+                   "max_inner": params["max_inner"], "warm_start": bool(warm)},
+        "plain_fim": plain,
+        "speedup_vs_plain": value / plain["value"],
+        "fim_stats_per_step": stats,
+        "roofline": roof,
+    }
+
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        ks = max(1, min(a.check_shots, len(src)))
+        sample = np.linspace(0, len(src) - 1, ks).round().astype(int)
+        _, Ts, _ = ctx.fim_solve(src[sample], rcv, return_T=True, **opts)
+        base, err = cpu_baseline_leg(case, picks, Ts, sample, threads)
+        out["cpu_baseline"] = base
+        out["speedup_vs_cpu"] = value / base["value"]
+        out["error_vs_fp64"] = err
+        out["error_gate_ok"] = err["max_pick_err_s"] <= 1e-4
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+"""Autotuner for the batched FIM launch shape.
+
+Chooses the block edge (4 or 8), the number of workgroups per round launch
+(which sets resident waves per CU: one 64-lane workgroup per visit slot) and
+the inner-iteration cap by timing the device solve (HIP events inside the
+library) on a sample of the caller's shots.  Results are cached per model
+shape and shot count for the life of the process.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+_CACHE = {}
+
+CUS = 256  # MI355X compute units
+
+
+def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=64, log=None,
+                candidates=None):
+    key = (ctx.shape, len(src), block, grid, max_inner)
+    if key in _CACHE:
+        return _CACHE[key]
+    if block and grid and max_inner:
+        p = dict(block=block, grid=grid, max_inner=max_inner)
+        _CACHE[key] = p
+        return p
+    n = min(sample, len(src))
+    idx = np.linspace(0, len(src) - 1, n).round().astype(int)
+    sub = np.ascontiguousarray(src[idx])
+    if candidates is None:
+        candidates = []
+        for B in ([block] if block else [4, 8]):
+            for wpc in ([grid // CUS] if grid else [8, 16, 32]):
+                for mi in ([max_inner] if max_inner else ([12, 16] if B == 4 else [6, 8])):
+                    candidates.append(dict(block=B, grid=wpc * CUS, max_inner=mi))
+    best, best_ms = None, float("inf")
+    for c in candidates:
+        ctx.fim_solve(sub[:2], rcv, batch=n, **c)  # first-touch of buffers
+        _, _, st = ctx.fim_solve(sub, rcv, batch=n, **c)
+        ms = st["solve_ms"]
+        if log:
+            log(f"tune {c}: {ms:.2f} ms for {n} shots, {st['visits']} visits")
+        if ms < best_ms:
+            best, best_ms = c, ms
+    _CACHE[key] = best
+    return best
+
+
+def load_surrogate(ctx, case, log=None):
+    """Loads the trained warm-start network into the context when one is
+    available (see seismic_tomography_3d_amd.surrogate); False otherwise."""
+    try:
+        from . import surrogate
+    except ImportError:
+        return False
+    return surrogate.attach(ctx, case, log=log)
