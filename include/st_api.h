@@ -92,6 +92,59 @@ int st_picks_volume(st_ctx *ctx, const float *T, int nx, int ny, int nz,
                     float dx, float dy, float dz, const float *rcv_xyz,
                     int nrcv, float *picks);
 
+
+/* ---- least-squares tomography (src/tomography/least_squares) ---------- */
+
+/* Inversion grid: n*_tomo = (int)((n-1)*d/d_tomo) + 1
+ * (least_squares.cpp:124-126).  max_steps caps each ray (0 = default);
+ * the reference's ray loop has none (:213-237). */
+int st_tomo_set_grid(st_ctx *ctx, float dx_tomo, float dy_tomo, float dz_tomo,
+                     int max_steps);
+int st_tomo_dims(st_ctx *ctx, int *nx_tomo, int *ny_tomo, int *nz_tomo);
+
+/* Least_squares::forward_modeling (least_squares.cpp:163-190): for every
+ * shot the eikonal solve, the receiver picks (dcal, nshot*nrcv) and the
+ * gradient ray tracing (:192-280) into the device-resident sensitivity
+ * matrix G (CSR, row = shot*nrcv + receiver, columns ascending).  illum
+ * (nullable, nx*ny*nz) receives the ray illumination summed over shots. */
+int st_forward(st_ctx *ctx, const float *src_xyz, int nshot,
+               const float *rcv_xyz, int nrcv, float *dcal, float *illum,
+               const st_fim_opts *opts, st_fim_stats *stats);
+
+/* Ray tracing on a caller volume T (z fastest, nx*ny*nz, the current model
+ * grid) for one source; rows are appended to G.  st_csr_reset clears G. */
+int st_csr_reset(st_ctx *ctx);
+int st_raytrace_volume(st_ctx *ctx, const float *T, const float *src_xyz,
+                       const float *rcv_xyz, int nrcv, float *illum);
+int st_csr_size(st_ctx *ctx, long long *nrows, long long *nnz);
+int st_csr_copy(st_ctx *ctx, long long *rowptr, int *cols, float *vals);
+
+/* converged() residual (least_squares.cpp:301-307), FP64 on the device */
+int st_residual_norm(st_ctx *ctx, const float *dobs, const float *dcal, int n,
+                     double *norm);
+
+/* Least_squares::optimization (least_squares.cpp:321-356): B = dobs - dcal
+ * padded with zeros, Tikhonov rows lambda*stencil (tk_reg_matrix,
+ * :358-422), CGLS (sparse_cgls, :424-496) for `iters` iterations with FP32
+ * vectors and FP64 dot products.  The solution stays on the device for
+ * st_model_update; xdm (nullable, n_model) receives a copy. */
+int st_optimize(st_ctx *ctx, const float *dobs, const float *dcal, int n,
+                int tk_order, float lambda, int iters, float *xdm);
+
+/* CGLS on a caller CSR matrix (N rows, M columns, rowptr has N+1 entries),
+ * x0 = 0: the sparse_cgls recurrence without regularisation rows. */
+int st_cgls_csr(st_ctx *ctx, const long long *rowptr, const int *cols,
+                const float *vals, int N, int M, const float *b, int iters,
+                float *x);
+
+/* Least_squares::model_update (least_squares.cpp:498-570) with the optional
+ * gaussian (src/utils/smoothing/gaussian.cpp:5-73): trilinear coarse-to-fine
+ * dm, smoothing, slowness += dm inside the margins.  xdm = NULL uses the
+ * device solution of st_optimize.  The device model is updated in place;
+ * dm and velocity (nullable, nx*ny*nz) receive copies. */
+int st_model_update(st_ctx *ctx, const float *xdm, int smooth, int samples,
+                    float stdv, float *dm, float *velocity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -597,46 +597,50 @@ static void fsm_inner(fsm_state *f)
     do { (f)->sgntz = tz; (f)->sgntx = tx; (f)->sgnty = ty; \
          (f)->sgnvz = vz; (f)->sgnvx = vx; (f)->sgnvy = vy; } while (0)
 
-/* init_sweep (accurate_FSM.cpp:267-388) */
+/* init_sweep (accurate_FSM.cpp:267-388).  The reverse loops start at
+ * sid+1; for a source on the last node of an axis the reference then reads
+ * one cell past the padded array (T[i - sgnt] with i = n+1).  The start is
+ * clamped to n here (identical results wherever the reference is defined). */
 static void fsm_init_sweep(fsm_state *f)
 {
     int nxx = f->nxx, nyy = f->nyy, nzz = f->nzz;
-    int sidx = f->sidx, sidy = f->sidy, sidz = f->sidz;
+    int sidx = imin(f->sidx, nxx - 3), sidy = imin(f->sidy, nyy - 3), sidz = imin(f->sidz, nzz - 3);
+    int fwdx = f->sidx, fwdy = f->sidy, fwdz = f->sidz;
 
     FSM_SET(f, 1, 1, 1, 1, 1, 1);
-    for (f->k = imax(1, sidy); f->k < nyy; f->k++)
-        for (f->j = imax(1, sidx); f->j < nxx; f->j++)
-            for (f->i = imax(1, sidz); f->i < nzz; f->i++) fsm_inner(f);
+    for (f->k = imax(1, fwdy); f->k < nyy; f->k++)
+        for (f->j = imax(1, fwdx); f->j < nxx; f->j++)
+            for (f->i = imax(1, fwdz); f->i < nzz; f->i++) fsm_inner(f);
 
     FSM_SET(f, -1, 1, 1, 0, 1, 1);
-    for (f->k = imax(1, sidy); f->k < nyy; f->k++)
-        for (f->j = imax(1, sidx); f->j < nxx; f->j++)
+    for (f->k = imax(1, fwdy); f->k < nyy; f->k++)
+        for (f->j = imax(1, fwdx); f->j < nxx; f->j++)
             for (f->i = sidz + 1; f->i >= 0; f->i--) fsm_inner(f);
 
     FSM_SET(f, 1, 1, -1, 1, 1, 0);
     for (f->k = sidy + 1; f->k >= 0; f->k--)
-        for (f->j = imax(1, sidx); f->j < nxx; f->j++)
-            for (f->i = imax(1, sidz); f->i < nzz; f->i++) fsm_inner(f);
+        for (f->j = imax(1, fwdx); f->j < nxx; f->j++)
+            for (f->i = imax(1, fwdz); f->i < nzz; f->i++) fsm_inner(f);
 
     FSM_SET(f, -1, 1, -1, 0, 1, 0);
     for (f->k = sidy + 1; f->k >= 0; f->k--)
-        for (f->j = imax(1, sidx); f->j < nxx; f->j++)
+        for (f->j = imax(1, fwdx); f->j < nxx; f->j++)
             for (f->i = sidz + 1; f->i >= 0; f->i--) fsm_inner(f);
 
     FSM_SET(f, 1, -1, 1, 1, 0, 1);
-    for (f->k = imax(1, sidy); f->k < nyy; f->k++)
+    for (f->k = imax(1, fwdy); f->k < nyy; f->k++)
         for (f->j = sidx + 1; f->j >= 0; f->j--)
-            for (f->i = imax(1, sidz); f->i < nzz; f->i++) fsm_inner(f);
+            for (f->i = imax(1, fwdz); f->i < nzz; f->i++) fsm_inner(f);
 
     FSM_SET(f, -1, -1, 1, 0, 0, 1);
-    for (f->k = imax(1, sidy); f->k < nyy; f->k++)
+    for (f->k = imax(1, fwdy); f->k < nyy; f->k++)
         for (f->j = sidx + 1; f->j >= 0; f->j--)
             for (f->i = sidz + 1; f->i >= 0; f->i--) fsm_inner(f);
 
     FSM_SET(f, 1, -1, -1, 1, 0, 0);
     for (f->k = sidy + 1; f->k >= 0; f->k--)
         for (f->j = sidx + 1; f->j >= 0; f->j--)
-            for (f->i = imax(1, sidz); f->i < nzz; f->i++) fsm_inner(f);
+            for (f->i = imax(1, fwdz); f->i < nzz; f->i++) fsm_inner(f);
 
     FSM_SET(f, -1, -1, -1, 0, 0, 0);
     for (f->k = sidy + 1; f->k >= 0; f->k--)

@@ -56,6 +56,25 @@ SIGNATURES = [
     ("st_picks_volume", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, c_float_p, ctypes.c_int, c_float_p]),
+    ("st_tomo_set_grid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_int]),
+    ("st_tomo_dims", ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, c_int_p]),
+    ("st_forward", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                  ctypes.c_int, c_float_p, c_float_p, ctypes.POINTER(FimOpts),
+                                  ctypes.POINTER(FimStats)]),
+    ("st_csr_reset", ctypes.c_int, [ctypes.c_void_p]),
+    ("st_raytrace_volume", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p,
+                                          ctypes.c_int, c_float_p]),
+    ("st_csr_size", ctypes.c_int, [ctypes.c_void_p, c_ll_p, c_ll_p]),
+    ("st_csr_copy", ctypes.c_int, [ctypes.c_void_p, c_ll_p, c_int_p, c_float_p]),
+    ("st_residual_norm", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]),
+    ("st_optimize", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_float, ctypes.c_int, c_float_p]),
+    ("st_cgls_csr", ctypes.c_int, [ctypes.c_void_p, c_ll_p, c_int_p, c_float_p, ctypes.c_int,
+                                   ctypes.c_int, c_float_p, ctypes.c_int, c_float_p]),
+    ("st_model_update", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_float, c_float_p, c_float_p]),
 ]
 
 _lib = None

@@ -108,3 +108,85 @@ class Context:
         self._check(self.lib.st_picks_volume(self.h, _ptr(T), nx, ny, nz, *spacing, _ptr(rcv),
                                              rcv.shape[0], _ptr(out)))
         return out
+
+    # ---- tomography ----
+    def tomo_set_grid(self, dxt, dyt=None, dzt=None, max_steps=0):
+        dyt = dxt if dyt is None else dyt
+        dzt = dxt if dzt is None else dzt
+        self._check(self.lib.st_tomo_set_grid(self.h, dxt, dyt, dzt, max_steps))
+        return self.tomo_dims()
+
+    def tomo_dims(self):
+        n = [ctypes.c_int() for _ in range(3)]
+        self._check(self.lib.st_tomo_dims(self.h, *[ctypes.byref(v) for v in n]))
+        return tuple(v.value for v in n)  # (nx_tomo, ny_tomo, nz_tomo)
+
+    def forward(self, src, rcv, illum=False, **opts):
+        """FIM + picks + ray tracing into the device G.  Returns (dcal, illum, stats)."""
+        src = _f32(src, (-1, 3))
+        rcv = _f32(rcv, (-1, 3))
+        dcal = np.empty((src.shape[0], rcv.shape[0]), np.float32)
+        il = np.empty(self.shape, np.float32) if illum else None
+        o = self.options(**opts)
+        st = FimStats()
+        self._check(self.lib.st_forward(self.h, _ptr(src), src.shape[0], _ptr(rcv), rcv.shape[0],
+                                        _ptr(dcal), _ptr(il), ctypes.byref(o), ctypes.byref(st)))
+        return dcal, il, st.as_dict()
+
+    def csr_reset(self):
+        self._check(self.lib.st_csr_reset(self.h))
+
+    def raytrace_volume(self, T, src, rcv, illum=False):
+        T = _f32(T)
+        src = _f32(src, (3,))
+        rcv = _f32(rcv, (-1, 3))
+        il = np.empty(self.shape, np.float32) if illum else None
+        self._check(self.lib.st_raytrace_volume(self.h, _ptr(T), _ptr(src), _ptr(rcv),
+                                                rcv.shape[0], _ptr(il)))
+        return il
+
+    def csr(self):
+        rows, nnz = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.st_csr_size(self.h, ctypes.byref(rows), ctypes.byref(nnz)))
+        rp = np.empty(rows.value + 1, np.int64)
+        c = np.empty(max(1, nnz.value), np.int32)
+        v = np.empty(max(1, nnz.value), np.float32)
+        self._check(self.lib.st_csr_copy(self.h, rp.ctypes.data_as(_lib.c_ll_p),
+                                         c.ctypes.data_as(_lib.c_int_p), _ptr(v)))
+        return rp, c[:nnz.value], v[:nnz.value]
+
+    def residual_norm(self, dobs, dcal):
+        dobs = _f32(dobs).ravel()
+        dcal = _f32(dcal).ravel()
+        out = ctypes.c_double()
+        self._check(self.lib.st_residual_norm(self.h, _ptr(dobs), _ptr(dcal), dobs.size,
+                                              ctypes.byref(out)))
+        return out.value
+
+    def optimize(self, dobs, dcal, tk_order=2, lam=1e-2, iters=10):
+        dobs = _f32(dobs).ravel()
+        dcal = _f32(dcal).ravel()
+        nt = self.tomo_dims()
+        x = np.empty(nt[0] * nt[1] * nt[2], np.float32)
+        self._check(self.lib.st_optimize(self.h, _ptr(dobs), _ptr(dcal), dobs.size, tk_order,
+                                         lam, iters, _ptr(x)))
+        return x
+
+    def cgls_csr(self, rowptr, cols, vals, M, b, iters):
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        cols = np.ascontiguousarray(cols, np.int32)
+        vals = _f32(vals)
+        b = _f32(b)
+        x = np.empty(M, np.float32)
+        self._check(self.lib.st_cgls_csr(self.h, rowptr.ctypes.data_as(_lib.c_ll_p),
+                                         cols.ctypes.data_as(_lib.c_int_p), _ptr(vals),
+                                         len(rowptr) - 1, M, _ptr(b), iters, _ptr(x)))
+        return x
+
+    def model_update(self, xdm=None, smooth=False, samples=5, stdv=2.0):
+        dm = np.empty(self.shape, np.float32)
+        vel = np.empty(self.shape, np.float32)
+        x = _f32(xdm) if xdm is not None else None
+        self._check(self.lib.st_model_update(self.h, _ptr(x), int(bool(smooth)), samples, stdv,
+                                             _ptr(dm), _ptr(vel)))
+        return dm, vel

@@ -21,6 +21,26 @@ bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float
 int surrogate_judge(st_ctx *ctx, const float *dsrc, int nshot, int B, float tol, int *bad);
 void tomo_release(st_ctx *ctx);
 void surrogate_release(st_ctx *ctx);
+struct TomoState;
+TomoState &tomo(st_ctx *ctx);
+bool tomo_grid(st_ctx *ctx, int *n, float *d, int *max_steps, int *max_runs);
+void tomo_set_grid(st_ctx *ctx, const int *n, const float *d, int max_steps, int max_runs);
+void tomo_reset(st_ctx *ctx, bool with_illum);
+void tomo_trace_blk(st_ctx *ctx, const float *T, long long vol, int B, int nbx, int nby,
+                    const float *dsrc, int nshot, const float *drcv, int nr, bool with_illum);
+void tomo_trace_zxy(st_ctx *ctx, const float *dT, const float *dsrc, const float *drcv, int nr,
+                    bool with_illum);
+void tomo_get_illum(st_ctx *ctx, float *out);
+void tomo_csr(st_ctx *ctx, long long *rows, long long *nnz, const long long **rp, const int **c,
+              const float **v);
+void tomo_load_csr(st_ctx *ctx, const long long *rowptr, const int *cols, const float *vals, int N,
+                   long long nnz);
+void tomo_set_rhs(st_ctx *ctx, const float *dobs, const float *dcal, int n, long long Ntot);
+void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters);
+const float *tomo_x(st_ctx *ctx);
+double tomo_residual(st_ctx *ctx, const float *dobs, const float *dcal, int n);
+void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, float stdv,
+                       float *dm_out, float *vel_out);
 
 template <class F> int guarded(st_ctx *ctx, F &&f) {
     try {
@@ -273,6 +293,244 @@ int st_picks_volume(st_ctx *ctx, const float *T, int nx, int ny, int nz, float d
         ST_HIP(hipMemcpyAsync(picks, ctx->picks.ptr, sizeof(float) * nrcv, hipMemcpyDeviceToHost,
                               ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+
+// ---- tomography --------------------------------------------------------
+
+int st_tomo_set_grid(st_ctx *ctx, float dxt, float dyt, float dzt, int max_steps) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(dxt > 0 && dyt > 0 && dzt > 0, ST_ERR_ARG, "bad inversion spacing");
+        int n[3];
+        n[2] = (int)((ctx->nz - 1) * ctx->dz / dzt) + 1;
+        n[0] = (int)((ctx->nx - 1) * ctx->dx / dxt) + 1;
+        n[1] = (int)((ctx->ny - 1) * ctx->dy / dyt) + 1;
+        const float d[3] = {dxt, dyt, dzt};
+        const float step = 0.2f * ctx->dz;
+        const float L = (ctx->nx + ctx->ny + ctx->nz) * std::max({ctx->dx, ctx->dy, ctx->dz});
+        if (max_steps <= 0) max_steps = (int)std::min(1e8f, 8.0f * L / step) + 64;
+        const int max_runs = 8 * (n[0] + n[1] + n[2]) + 64;
+        tomo_set_grid(ctx, n, d, max_steps, max_runs);
+    });
+}
+
+int st_tomo_dims(st_ctx *ctx, int *nxt, int *nyt, int *nzt) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        int n[3], ms, mr;
+        float d[3];
+        ST_REQUIRE(tomo_grid(ctx, n, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+        if (nxt) *nxt = n[0];
+        if (nyt) *nyt = n[1];
+        if (nzt) *nzt = n[2];
+    });
+}
+
+int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz, int nrcv,
+               float *dcal, float *illum, const st_fim_opts *opts, st_fim_stats *stats) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        int n[3], ms, mr;
+        float d[3];
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(tomo_grid(ctx, n, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+        ST_REQUIRE(nshot > 0 && nrcv > 0 && src_xyz && rcv_xyz && dcal, ST_ERR_ARG,
+                   "forward needs sources, receivers and a dcal buffer");
+        st_fim_opts o;
+        if (opts) o = *opts; else st_fim_default_opts(&o);
+        ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
+        if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 16 : 16;
+        if (o.eps < 0.f) o.eps = 0.f;
+        ST_REQUIRE(o.warm == 0, ST_ERR_ARG, "forward runs cold starts");
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        check_points(src_xyz, nshot, ctx, "source");
+        check_points(rcv_xyz, nrcv, ctx, "receiver");
+        set_device(ctx);
+        const int B = o.block;
+        ensure_block_model(ctx, B);
+        const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+        int batch = std::max(1, std::min({o.batch, nshot, 1023}));
+        ctx->T.reserve(sizeof(float) * g.vol * batch);
+        ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
+        ctx->list[0].reserve(sizeof(unsigned) * g.nblk * batch);
+        ctx->list[1].reserve(sizeof(unsigned) * g.nblk * batch);
+        ctx->src.reserve(sizeof(float) * 3 * nshot);
+        ctx->rcv.reserve(sizeof(float) * 3 * nrcv);
+        ctx->picks.reserve(sizeof(float) * (long long)nshot * nrcv);
+        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
+                              hipMemcpyHostToDevice, ctx->stream));
+        ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
+                              hipMemcpyHostToDevice, ctx->stream));
+        tomo_reset(ctx, true);
+        for (int b0 = 0; b0 < nshot; b0 += batch) {
+            const int nb = std::min(batch, nshot - b0);
+            const float *dsrc = ctx->src.as<float>() + 3LL * b0;
+            fim_batch(ctx, dsrc, nb, o, stats, false);
+            launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, ctx->rcv.as<float>(), nrcv,
+                             ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx, ctx->ny,
+                             ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
+            tomo_trace_blk(ctx, ctx->T.as<float>(), g.vol, B, g.nbx, g.nby, dsrc, nb,
+                           ctx->rcv.as<float>(), nrcv, true);
+        }
+        ST_HIP(hipMemcpyAsync(dcal, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        if (illum) tomo_get_illum(ctx, illum);
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int st_csr_reset(st_ctx *ctx) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        set_device(ctx);
+        tomo_reset(ctx, true);
+    });
+}
+
+int st_raytrace_volume(st_ctx *ctx, const float *T, const float *src_xyz, const float *rcv_xyz,
+                       int nrcv, float *illum) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        int n[3], ms, mr;
+        float d[3];
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(tomo_grid(ctx, n, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+        ST_REQUIRE(T && src_xyz && rcv_xyz && nrcv > 0, ST_ERR_ARG, "bad arguments");
+        check_points(src_xyz, 1, ctx, "source");
+        check_points(rcv_xyz, nrcv, ctx, "receiver");
+        set_device(ctx);
+        const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+        ctx->scratch.reserve(sizeof(float) * npts);
+        ctx->src.reserve(sizeof(float) * 3);
+        ctx->rcv.reserve(sizeof(float) * 3 * nrcv);
+        ST_HIP(hipMemcpyAsync(ctx->scratch.ptr, T, sizeof(float) * npts, hipMemcpyHostToDevice,
+                              ctx->stream));
+        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3, hipMemcpyHostToDevice,
+                              ctx->stream));
+        ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
+                              hipMemcpyHostToDevice, ctx->stream));
+        tomo_trace_zxy(ctx, ctx->scratch.as<float>(), ctx->src.as<float>(), ctx->rcv.as<float>(),
+                       nrcv, true);
+        if (illum) tomo_get_illum(ctx, illum);
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int st_csr_size(st_ctx *ctx, long long *nrows, long long *nnz) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        const long long *rp;
+        const int *c;
+        const float *v;
+        long long r = 0, z = 0;
+        tomo_csr(ctx, &r, &z, &rp, &c, &v);
+        if (nrows) *nrows = r;
+        if (nnz) *nnz = z;
+    });
+}
+
+int st_csr_copy(st_ctx *ctx, long long *rowptr, int *cols, float *vals) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        const long long *rp;
+        const int *c;
+        const float *v;
+        long long r = 0, z = 0;
+        tomo_csr(ctx, &r, &z, &rp, &c, &v);
+        set_device(ctx);
+        if (rowptr && r >= 0)
+            ST_HIP(hipMemcpyAsync(rowptr, rp, sizeof(long long) * (r + 1), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        if (cols && z)
+            ST_HIP(hipMemcpyAsync(cols, c, sizeof(int) * z, hipMemcpyDeviceToHost, ctx->stream));
+        if (vals && z)
+            ST_HIP(hipMemcpyAsync(vals, v, sizeof(float) * z, hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int st_residual_norm(st_ctx *ctx, const float *dobs, const float *dcal, int n, double *norm) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(dobs && dcal && norm && n > 0, ST_ERR_ARG, "bad arguments");
+        set_device(ctx);
+        *norm = tomo_residual(ctx, dobs, dcal, n);
+    });
+}
+
+int st_optimize(st_ctx *ctx, const float *dobs, const float *dcal, int n, int tk_order,
+                float lambda, int iters, float *xdm) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        int nt[3], ms, mr;
+        float d[3];
+        ST_REQUIRE(tomo_grid(ctx, nt, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+        long long rows = 0, nnz = 0;
+        const long long *rp;
+        const int *c;
+        const float *v;
+        tomo_csr(ctx, &rows, &nnz, &rp, &c, &v);
+        ST_REQUIRE(rows > 0 && rows == n, ST_ERR_STATE,
+                   "st_optimize needs the G of st_forward with n = shots*receivers rows");
+        ST_REQUIRE(dobs && dcal && iters >= 0 && tk_order >= 0, ST_ERR_ARG, "bad arguments");
+        set_device(ctx);
+        const int n_model = nt[0] * nt[1] * nt[2];
+        tomo_set_rhs(ctx, dobs, dcal, n, rows + n_model - tk_order);
+        tomo_cgls(ctx, n_model, tk_order, lambda, iters);
+        if (xdm) {
+            ST_HIP(hipMemcpyAsync(xdm, tomo_x(ctx), sizeof(float) * n_model, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            ST_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int st_cgls_csr(st_ctx *ctx, const long long *rowptr, const int *cols, const float *vals, int N,
+                int M, const float *b, int iters, float *x) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(rowptr && cols && vals && b && x && N > 0 && M > 0 && iters >= 0, ST_ERR_ARG,
+                   "bad arguments");
+        const long long nnz = rowptr[N];
+        ST_REQUIRE(rowptr[0] == 0 && nnz >= 0, ST_ERR_ARG, "bad rowptr");
+        for (long long e = 0; e < nnz; e++)
+            ST_REQUIRE(cols[e] >= 0 && cols[e] < M, ST_ERR_ARG, "column out of range");
+        set_device(ctx);
+        tomo_load_csr(ctx, rowptr, cols, vals, N, nnz);
+        // lambda = 0 order-0 rows: zero rows that leave A^T A unchanged
+        tomo_set_rhs(ctx, b, nullptr, N, (long long)N + M);
+        tomo_cgls(ctx, M, 0, 0.0f, iters);
+        ST_HIP(hipMemcpyAsync(x, tomo_x(ctx), sizeof(float) * M, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int st_model_update(st_ctx *ctx, const float *xdm, int smooth, int samples, float stdv, float *dm,
+                    float *velocity) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        int nt[3], ms, mr;
+        float d[3];
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(tomo_grid(ctx, nt, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+        set_device(ctx);
+        const float *dx = nullptr;
+        if (xdm) {
+            const long long n_model = (long long)nt[0] * nt[1] * nt[2];
+            ctx->scratch.reserve(sizeof(float) * n_model);
+            ST_HIP(hipMemcpyAsync(ctx->scratch.ptr, xdm, sizeof(float) * n_model,
+                                  hipMemcpyHostToDevice, ctx->stream));
+            dx = ctx->scratch.as<float>();
+        } else {
+            dx = tomo_x(ctx);
+            ST_REQUIRE(dx, ST_ERR_STATE, "no solution: call st_optimize or pass xdm");
+        }
+        tomo_model_update(ctx, dx, smooth, samples, stdv, dm, velocity);
     });
 }
 

@@ -1,6 +1,876 @@
-// tomo.hip -- ray tracing, CGLS and model update (filled in next).
+// tomo.hip -- least-squares traveltime tomography on the device.
+//
+// Replaces, in src/tomography/least_squares/least_squares.cpp:
+//   expand_fdm (:11-67)       -- not materialised: the tracer reads the FIM
+//                                volume through a clamp-to-edge index map,
+//                                which is what the one-cell replication pad is;
+//   ray_tracing (:192-280)    -- one thread per ray, count -> scan -> fill CSR;
+//   optimization (:321-356),
+//   tk_reg_matrix (:358-422)  -- Tikhonov rows appended on the device;
+//   sparse_cgls (:424-496)    -- CSR SpMV, SpMV^T from an explicit transposed
+//                                matrix (deterministic, no atomics), FP64 dots;
+//   converged (:301-319)      -- FP64 residual norm;
+//   model_update (:498-570),
+//   gaussian (src/utils/smoothing/gaussian.cpp:5-73).
+//
+// Compiled with -ffp-contract=off and correctly rounded division/sqrt: the
+// tracer, the model update and the smoothing reproduce the CPU formulas bit
+// for bit on the same input volume.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "st_common.hpp"
 
 namespace st {
-void tomo_release(st_ctx *) {}
+
+// ---------------------------------------------------------------------------
+// state
+// ---------------------------------------------------------------------------
+struct TomoState {
+    bool grid_set = false;
+    int nxt = 0, nyt = 0, nzt = 0;
+    float dxt = 0, dyt = 0, dzt = 0;
+    int max_steps = 0, max_runs = 0;
+
+    // sensitivity matrix G (CSR), grown batch by batch
+    DevBuf rowptr, cols, vals;     // int64 rows+1, int32, float
+    long long rows = 0, nnz = 0;
+    long long cap_rows = 0, cap_nnz = 0;
+
+    // per-ray scratch of the current batch
+    DevBuf runs, nruns, fdist, rowlen, scanbuf;
+    DevBuf err;                    // unsigned error word
+    DevBuf illum;                  // nx*ny*nz
+    bool illum_valid = false;
+
+    // least squares
+    int n_model = 0, n_data = 0, order = 0, N = 0, M = 0;
+    long long annz = 0;
+    DevBuf t_colptr, t_rows, t_vals, t_cursor;
+    DevBuf b, s, q, r, p, x, dots, scal;
+    bool have_x = false;
+
+    DevBuf dm, upd, kern;
+};
+
+TomoState &tomo(st_ctx *ctx) {
+    if (!ctx->tomo) ctx->tomo = new TomoState();
+    return *static_cast<TomoState *>(ctx->tomo);
+}
+
+void tomo_release(st_ctx *ctx) {
+    if (!ctx->tomo) return;
+    TomoState &t = *static_cast<TomoState *>(ctx->tomo);
+    for (DevBuf *b : {&t.rowptr, &t.cols, &t.vals, &t.runs, &t.nruns, &t.fdist, &t.rowlen,
+                      &t.scanbuf, &t.err, &t.illum, &t.t_colptr, &t.t_rows, &t.t_vals,
+                      &t.t_cursor, &t.b, &t.s, &t.q, &t.r, &t.p, &t.x, &t.dots, &t.scal, &t.dm,
+                      &t.upd, &t.kern})
+        b->release();
+    delete &t;
+    ctx->tomo = nullptr;
+}
+
+// grow a device buffer keeping its first `keep` bytes
+static void grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st) {
+    if (need <= b.bytes) return;
+    size_t nb = std::max(need, b.bytes + b.bytes / 2);
+    void *np = nullptr;
+    ST_HIP(hipMalloc(&np, nb));
+    if (keep && b.ptr) ST_HIP(hipMemcpyAsync(np, b.ptr, keep, hipMemcpyDeviceToDevice, st));
+    ST_HIP(hipStreamSynchronize(st));
+    if (b.ptr) ST_HIP(hipFree(b.ptr));
+    b.ptr = np;
+    b.bytes = nb;
+}
+
+// ---------------------------------------------------------------------------
+// ray tracing
+// ---------------------------------------------------------------------------
+struct ZxyLay {
+    int nx, nz;
+    __device__ long long operator()(int x, int y, int z) const {
+        return z + (long long)x * nz + (long long)y * nx * nz;
+    }
+};
+struct BlkLay {
+    int B, nbx, nby;
+    __device__ long long operator()(int x, int y, int z) const {
+        return blk_addr(x, y, z, B, nbx, nby);
+    }
+};
+
+struct RayArgs {
+    const float *T;
+    long long vol;
+    int nx, ny, nz;
+    float dx, dy, dz;
+    int nxt, nyt, nzt;
+    float dxt, dyt, dzt;
+    const float *src;   // 3 per shot of this batch
+    const float *rcv;   // 3 per receiver
+    int nr, nshot;
+    int max_steps, max_runs;
+    int2 *runs;         // (cell, count) per ray, max_runs each
+    int *nruns;         // merged entries per ray
+    float *fdist;       // finalDist per ray
+    float *illum;       // nullable
+    unsigned *err;      // bit 0: step cap / lost gradient, bit 1: run capacity
+};
+
+__device__ __forceinline__ int iclampd(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <class L>
+__global__ void ray_trace(RayArgs a, L lay) {
+    const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= (long long)a.nshot * a.nr) return;
+    const int s = (int)(ray / a.nr), r = (int)(ray - (long long)s * a.nr);
+    const float *Ts = a.T + (long long)s * a.vol;
+    const float sx = a.src[3 * s], sy = a.src[3 * s + 1], sz = a.src[3 * s + 2];
+    const int sIdz = (int)(sz / a.dzt), sIdx = (int)(sx / a.dxt), sIdy = (int)(sy / a.dyt);
+    const int sId = sIdz + sIdx * a.nzt + sIdy * a.nxt * a.nzt;
+    const float rayStep = 0.2f * a.dz;
+    // expand_fdm pad: padded index p maps to model index clamp(p - 1)
+    auto Tp = [&](int i, int j, int k) -> float {
+        return Ts[lay(iclampd(j - 1, 0, a.nx - 1), iclampd(k - 1, 0, a.ny - 1),
+                      iclampd(i - 1, 0, a.nz - 1))];
+    };
+    float zi = a.rcv[3 * r + 2], xi = a.rcv[3 * r], yi = a.rcv[3 * r + 1];
+    int2 *my = a.runs + ray * a.max_runs;
+    int nrun = 0, cur = -1, cnt = 0;
+    long long il_cur = -1;
+    float il_acc = 0.f;
+    const long long npts = (long long)a.nx * a.ny * a.nz;
+    bool ok = true;
+    for (int step = 0;; step++) {
+        if (step >= a.max_steps) { ok = false; atomicOr(a.err, 1u); break; }
+        const int i = iclampd((int)(zi / a.dz) + 1, 1, a.nz);
+        const int j = iclampd((int)(xi / a.dx) + 1, 1, a.nx);
+        const int k = iclampd((int)(yi / a.dy) + 1, 1, a.ny);
+        const float dTz = (Tp(i + 1, j, k) - Tp(i - 1, j, k)) / (2.0f * a.dz);
+        const float dTx = (Tp(i, j + 1, k) - Tp(i, j - 1, k)) / (2.0f * a.dx);
+        const float dTy = (Tp(i, j, k + 1) - Tp(i, j, k - 1)) / (2.0f * a.dy);
+        const float norm = sqrtf(dTx * dTx + dTy * dTy + dTz * dTz);
+        if (!(norm > 0.0f) || !isfinite(norm)) { ok = false; atomicOr(a.err, 1u); break; }
+        zi -= rayStep * dTz / norm;
+        xi -= rayStep * dTx / norm;
+        yi -= rayStep * dTy / norm;
+        const int im = iclampd((int)(zi / a.dzt), 0, a.nzt - 1);
+        const int jm = iclampd((int)(xi / a.dxt), 0, a.nxt - 1);
+        const int km = iclampd((int)(yi / a.dyt), 0, a.nyt - 1);
+        const int cell = im + jm * a.nzt + km * a.nxt * a.nzt;
+        if (cell != cur) {
+            if (cnt) {
+                if (nrun >= a.max_runs) { ok = false; atomicOr(a.err, 2u); break; }
+                my[nrun++] = make_int2(cur, cnt);
+            }
+            cur = cell;
+            cnt = 0;
+        }
+        cnt++;
+        // illumination at the pre-step node with the reference's strides
+        // (least_squares.cpp:234); writes outside the volume are dropped
+        const long long il = i + (long long)j * a.nz + (long long)k * a.nx * a.nz;
+        if (il != il_cur) {
+            if (a.illum && il_cur >= 0 && il_cur < npts) atomicAdd(&a.illum[il_cur], il_acc);
+            il_cur = il;
+            il_acc = 0.f;
+        }
+        il_acc += rayStep;
+        if (cell == sId) break;
+    }
+    if (a.illum && il_cur >= 0 && il_cur < npts) atomicAdd(&a.illum[il_cur], il_acc);
+    if (!ok) { a.nruns[ray] = 0; a.fdist[ray] = 0.f; return; }
+    if (nrun >= a.max_runs) { atomicOr(a.err, 2u); a.nruns[ray] = 0; return; }
+    my[nrun++] = make_int2(cur, cnt);
+    a.fdist[ray] = sqrtf((zi - sz) * (zi - sz) + (xi - sx) * (xi - sx) + (yi - sy) * (yi - sy));
+
+    // sort the runs by cell (shell sort, Ciura gaps) and merge repeats: the
+    // std::sort + run-length pass of least_squares.cpp:241-276
+    const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+    for (int gi = 0; gi < 8; gi++) {
+        const int gap = gaps[gi];
+        for (int u = gap; u < nrun; u++) {
+            const int2 tmp = my[u];
+            int w = u;
+            while (w >= gap && my[w - gap].x > tmp.x) { my[w] = my[w - gap]; w -= gap; }
+            my[w] = tmp;
+        }
+    }
+    int m = 0;
+    for (int u = 0; u < nrun; u++) {
+        if (m > 0 && my[m - 1].x == my[u].x) my[m - 1].y += my[u].y;
+        else my[m++] = my[u];
+    }
+    a.nruns[ray] = m;
+}
+
+// CSR values: first (smallest) cell gets rayStep + n additions, later cells
+// rayStep + (n-1) additions (sequential FP32 sums), the source cell finalDist.
+__global__ void ray_fill(const int2 *__restrict__ runs, const int *__restrict__ nruns,
+                         const float *__restrict__ fdist, const long long *__restrict__ rowptr,
+                         int *__restrict__ cols, float *__restrict__ vals, long long nrays,
+                         int max_runs, const float *__restrict__ src, int nr, float dz, float dxt,
+                         float dyt, float dzt, int nxt, int nzt) {
+    const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= nrays) return;
+    const int s = (int)(ray / nr);
+    const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+    const int sId = (int)(sz / dzt) + (int)(sx / dxt) * nzt + (int)(sy / dyt) * nxt * nzt;
+    const float rayStep = 0.2f * dz;
+    const int2 *my = runs + ray * max_runs;
+    const long long o = rowptr[ray];
+    const int m = nruns[ray];
+    for (int u = 0; u < m; u++) {
+        const int n = my[u].y;
+        float d = rayStep;
+        const int adds = (u == 0) ? n : n - 1;
+        for (int q = 0; q < adds; q++) d += rayStep;
+        cols[o + u] = my[u].x;
+        vals[o + u] = (my[u].x == sId) ? fdist[ray] : d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scan (exclusive, int -> int64), three phases
+// ---------------------------------------------------------------------------
+constexpr int kScanTile = 1024;
+
+__global__ void scan_tiles(const int *__restrict__ in, long long *__restrict__ out,
+                           long long *__restrict__ tile_sums, long long n) {
+    __shared__ long long sh[kScanTile];
+    const long long base = (long long)blockIdx.x * kScanTile;
+    const int t = threadIdx.x;  // 256 threads, 4 items each
+    long long v[4];
+    long long acc = 0;
+    for (int k = 0; k < 4; k++) {
+        const long long i = base + t * 4 + k;
+        v[k] = (i < n) ? in[i] : 0;
+        acc += v[k];
+    }
+    sh[t] = acc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        long long add = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    long long run = sh[t] - acc;
+    for (int k = 0; k < 4; k++) {
+        const long long i = base + t * 4 + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (t == 255) tile_sums[blockIdx.x] = sh[255];
+}
+
+__global__ void scan_sums(long long *tile_sums, int ntiles, long long *total) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    long long run = 0;
+    for (int i = 0; i < ntiles; i++) {
+        const long long v = tile_sums[i];
+        tile_sums[i] = run;
+        run += v;
+    }
+    *total = run;
+}
+
+__global__ void scan_add(long long *out, const long long *tile_sums, long long n, long long offset) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += tile_sums[i / kScanTile] + offset;
+}
+
+// exclusive scan of n ints into out[0..n) plus offset; returns the total.
+static long long exclusive_scan(st_ctx *ctx, const int *in, long long *out, long long n,
+                                long long offset, DevBuf &tmp) {
+    const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
+    tmp.reserve(sizeof(long long) * (ntiles + 2));
+    long long *sums = tmp.as<long long>();
+    long long *total = sums + ntiles;
+    hipLaunchKernelGGL(scan_tiles, dim3(ntiles), dim3(256), 0, ctx->stream, in, out, sums, n);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1), 0, ctx->stream, sums, ntiles, total);
+    hipLaunchKernelGGL(scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, out,
+                       sums, n, offset);
+    ST_HIP(hipGetLastError());
+    long long *h = static_cast<long long *>(ctx->host_pinned) + 64;
+    ST_HIP(hipMemcpyAsync(h, total, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    return *h;
+}
+
+// Traces nshot*nr rays on T (layout lay) and appends their rows to G.
+template <class L>
+static void trace_append(st_ctx *ctx, const float *T, long long vol, L lay, const float *dsrc,
+                         int nshot, const float *drcv, int nr, bool with_illum) {
+    TomoState &t = tomo(ctx);
+    const long long nrays = (long long)nshot * nr;
+    t.runs.reserve(sizeof(int2) * nrays * t.max_runs);
+    t.nruns.reserve(sizeof(int) * nrays);
+    t.fdist.reserve(sizeof(float) * nrays);
+    t.err.reserve(64);
+    ST_HIP(hipMemsetAsync(t.err.ptr, 0, 4, ctx->stream));
+    RayArgs a;
+    a.T = T; a.vol = vol;
+    a.nx = ctx->nx; a.ny = ctx->ny; a.nz = ctx->nz;
+    a.dx = ctx->dx; a.dy = ctx->dy; a.dz = ctx->dz;
+    a.nxt = t.nxt; a.nyt = t.nyt; a.nzt = t.nzt;
+    a.dxt = t.dxt; a.dyt = t.dyt; a.dzt = t.dzt;
+    a.src = dsrc; a.rcv = drcv; a.nr = nr; a.nshot = nshot;
+    a.max_steps = t.max_steps; a.max_runs = t.max_runs;
+    a.runs = t.runs.as<int2>(); a.nruns = t.nruns.as<int>(); a.fdist = t.fdist.as<float>();
+    a.illum = with_illum ? t.illum.as<float>() : nullptr;
+    a.err = t.err.as<unsigned>();
+    hipLaunchKernelGGL(ray_trace<L>, dim3((unsigned)((nrays + 63) / 64)), dim3(64), 0, ctx->stream,
+                       a, lay);
+    ST_HIP(hipGetLastError());
+    unsigned *herr = static_cast<unsigned *>(ctx->host_pinned) + 200;
+    ST_HIP(hipMemcpyAsync(herr, t.err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    ST_REQUIRE(!(*herr & 1u), ST_ERR_DIVERGED,
+               "a ray exceeded its step cap or lost its gradient before reaching the source cell");
+    ST_REQUIRE(!(*herr & 2u), ST_ERR_CAPACITY, "a ray crossed more cells than max_runs");
+
+    // rows of this batch start at t.rows in the global rowptr
+    grow_keep(t.rowptr, sizeof(long long) * (t.rows + nrays + 1), sizeof(long long) * (t.rows + 1),
+              ctx->stream);
+    long long *rp = t.rowptr.as<long long>() + t.rows;
+    const long long add = exclusive_scan(ctx, t.nruns.as<int>(), rp, nrays, t.nnz, t.scanbuf);
+    const long long new_nnz = t.nnz + add;
+    ST_HIP(hipMemcpyAsync(rp + nrays, &new_nnz, sizeof(long long), hipMemcpyHostToDevice,
+                          ctx->stream));
+    grow_keep(t.cols, sizeof(int) * new_nnz, sizeof(int) * t.nnz, ctx->stream);
+    grow_keep(t.vals, sizeof(float) * new_nnz, sizeof(float) * t.nnz, ctx->stream);
+    hipLaunchKernelGGL(ray_fill, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, ctx->stream,
+                       t.runs.as<int2>(), t.nruns.as<int>(), t.fdist.as<float>(), rp,
+                       t.cols.as<int>(), t.vals.as<float>(), nrays, t.max_runs, dsrc, nr, ctx->dz,
+                       t.dxt, t.dyt, t.dzt, t.nxt, t.nzt);
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    t.rows += nrays;
+    t.nnz = new_nnz;
+}
+
+void tomo_trace_blk(st_ctx *ctx, const float *T, long long vol, int B, int nbx, int nby,
+                    const float *dsrc, int nshot, const float *drcv, int nr, bool with_illum) {
+    trace_append(ctx, T, vol, BlkLay{B, nbx, nby}, dsrc, nshot, drcv, nr, with_illum);
+}
+
+// ---------------------------------------------------------------------------
+// least squares: Tikhonov rows, transpose, CGLS
+// ---------------------------------------------------------------------------
+
+// rows n_data .. n_data + n_model - order - 1 (tk_reg_matrix, :397-417)
+__global__ void tikhonov_rows(long long *rowptr, int *cols, float *vals, long long row0,
+                              long long nnz0, int nrow, int order, const float *w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    const int e = order + 1;
+    const long long o = nnz0 + (long long)i * e;
+    rowptr[row0 + i] = o;
+    if (i == nrow - 1) rowptr[row0 + nrow] = o + e;
+    for (int k = 0; k < e; k++) {
+        cols[o + k] = i + k;
+        vals[o + k] = w[k];
+    }
+}
+
+__global__ void col_count(const int *__restrict__ cols, long long nnz, int *__restrict__ cnt) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) atomicAdd(&cnt[cols[e]], 1);
+}
+
+// row index of each entry by binary search of rowptr, then scatter into the
+// column's slot; the order inside a column is fixed afterwards by col_sort
+__global__ void col_scatter(const long long *__restrict__ rowptr, int nrows,
+                            const int *__restrict__ cols, const float *__restrict__ vals,
+                            long long nnz, const long long *__restrict__ colptr,
+                            int *__restrict__ cursor, int *__restrict__ trow,
+                            float *__restrict__ tval) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {  // last row with rowptr[row] <= e
+        const int mid = (lo + hi + 1) >> 1;
+        if (rowptr[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    const int c = cols[e];
+    const long long pos = colptr[c] + atomicAdd(&cursor[c], 1);
+    trow[pos] = lo;
+    tval[pos] = vals[e];
+}
+
+// sort each column's (row, value) pairs by row: deterministic SpMV^T order
+__global__ void col_sort(const long long *__restrict__ colptr, int ncol, int *__restrict__ trow,
+                         float *__restrict__ tval) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    const long long b = colptr[c], n = colptr[c + 1] - b;
+    int *R = trow + b;
+    float *V = tval + b;
+    const int gaps[10] = {4001, 1750, 701, 301, 132, 57, 23, 10, 4, 1};
+    for (int gi = 0; gi < 10; gi++) {
+        const int gap = gaps[gi];
+        for (long long u = gap; u < n; u++) {
+            const int tr = R[u];
+            const float tv = V[u];
+            long long w = u;
+            while (w >= gap && R[w - gap] > tr) { R[w] = R[w - gap]; V[w] = V[w - gap]; w -= gap; }
+            R[w] = tr;
+            V[w] = tv;
+        }
+    }
+}
+
+// y[i] = sum_e A[i, col[e]] x[col[e]]: 16 lanes per row, fixed reduction tree
+__global__ void spmv_csr(const long long *__restrict__ rowptr, const int *__restrict__ idx,
+                         const float *__restrict__ val, const float *__restrict__ x,
+                         float *__restrict__ y, int nrows) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long row = gid >> 4;
+    const int l = threadIdx.x & 15;
+    float acc = 0.f;
+    if (row < nrows) {
+        const long long b = rowptr[row], e = rowptr[row + 1];
+        for (long long k = b + l; k < e; k += 16) acc += val[k] * x[idx[k]];
+    }
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (row < nrows && l == 0) y[row] = acc;
+}
+
+// FP64 dot products: per-block partials, then one ordered pass
+__global__ void dot_partial(const float *__restrict__ a, const float *__restrict__ b, long long n,
+                            double *__restrict__ part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        acc += (double)a[i] * (double)b[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void dot_final(const double *__restrict__ part, int n, double *__restrict__ out) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += part[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+constexpr int kDotBlocks = 512;
+
+static void dot(st_ctx *ctx, const float *a, const float *b, long long n, double *part, double *out) {
+    hipLaunchKernelGGL(dot_partial, dim3(kDotBlocks), dim3(256), 0, ctx->stream, a, b, n, part);
+    hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, ctx->stream, part, kDotBlocks, out);
+}
+
+// scal[0] = num / den (FP64 ratio, used as FP32 step)
+// (a zero denominator -- an exactly solved system -- gives a zero step
+// where the reference would produce NaN)
+__global__ void ratio(const double *num, const double *den, float *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = (*den != 0.0) ? (float)(*num / *den) : 0.0f;
+}
+
+// y = y + sign * alpha * x
+__global__ void axpy(float *__restrict__ y, const float *__restrict__ x, const float *alpha,
+                     float sign, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += sign * (*alpha) * x[i];
+}
+
+// p = r + beta * p
+__global__ void xpby(float *__restrict__ p, const float *__restrict__ r, const float *beta, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = r[i] + (*beta) * p[i];
+}
+
+__global__ void sub_vec(const float *a, const float *b, float *out, long long n, long long ntot) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+    else if (i < ntot) out[i] = 0.f;
+}
+
+static unsigned nblk256(long long n) { return (unsigned)((n + 255) / 256); }
+
+// Assembles A = [G; lambda*L] in place after G, builds A^T, runs CGLS.
+void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
+    TomoState &t = tomo(ctx);
+    const long long n_data = t.rows;
+    ST_REQUIRE(order >= 0 && order < n_model, ST_ERR_ARG, "bad Tikhonov order");
+    const int nL = n_model - order;
+    const long long N = n_data + nL;
+    const long long annz = t.nnz + (long long)(order + 1) * nL;
+    // stencil (least_squares.cpp:369-410): order 0 -> [1], else repeated
+    // differences of [-1, 1]
+    std::vector<int> df(order + 2, 0), d1(order + 3, 0), d2(order + 3, 0);
+    if (order == 0) {
+        df[0] = 1;
+    } else {
+        df[0] = -1; df[1] = 1;
+        for (int index = 1; index < order; index++)
+            for (int k = 0; k < order + 1; k++) {
+                d2[k] = df[k];
+                d1[k + 1] = df[k];
+                df[k] = d1[k] - d2[k];
+            }
+    }
+    float *w = static_cast<float *>(ctx->host_pinned) + 300;
+    for (int k = 0; k <= order; k++) w[k] = lambda * (float)df[k];
+
+    grow_keep(t.rowptr, sizeof(long long) * (N + 1), sizeof(long long) * (n_data + 1), ctx->stream);
+    grow_keep(t.cols, sizeof(int) * annz, sizeof(int) * t.nnz, ctx->stream);
+    grow_keep(t.vals, sizeof(float) * annz, sizeof(float) * t.nnz, ctx->stream);
+    t.kern.reserve(sizeof(float) * 64);
+    ST_HIP(hipMemcpyAsync(t.kern.ptr, w, sizeof(float) * (order + 1), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(tikhonov_rows, dim3(nblk256(nL)), dim3(256), 0, ctx->stream,
+                       t.rowptr.as<long long>(), t.cols.as<int>(), t.vals.as<float>(), n_data,
+                       t.nnz, nL, order, t.kern.as<float>());
+
+    // A^T (CSC of A), rows ascending inside each column
+    t.t_colptr.reserve(sizeof(long long) * (n_model + 1));
+    t.t_cursor.reserve(sizeof(int) * (n_model + 1));
+    t.t_rows.reserve(sizeof(int) * annz);
+    t.t_vals.reserve(sizeof(float) * annz);
+    ST_HIP(hipMemsetAsync(t.t_cursor.ptr, 0, sizeof(int) * (n_model + 1), ctx->stream));
+    hipLaunchKernelGGL(col_count, dim3(nblk256(annz)), dim3(256), 0, ctx->stream, t.cols.as<int>(),
+                       annz, t.t_cursor.as<int>());
+    const long long tot = exclusive_scan(ctx, t.t_cursor.as<int>(), t.t_colptr.as<long long>(),
+                                         n_model, 0, t.scanbuf);
+    ST_REQUIRE(tot == annz, ST_ERR_HIP, "transpose count mismatch");
+    ST_HIP(hipMemcpyAsync(t.t_colptr.as<long long>() + n_model, &tot, sizeof(long long),
+                          hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemsetAsync(t.t_cursor.ptr, 0, sizeof(int) * (n_model + 1), ctx->stream));
+    hipLaunchKernelGGL(col_scatter, dim3(nblk256(annz)), dim3(256), 0, ctx->stream,
+                       t.rowptr.as<long long>(), (int)N, t.cols.as<int>(), t.vals.as<float>(), annz,
+                       t.t_colptr.as<long long>(), t.t_cursor.as<int>(), t.t_rows.as<int>(),
+                       t.t_vals.as<float>());
+    hipLaunchKernelGGL(col_sort, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream,
+                       t.t_colptr.as<long long>(), n_model, t.t_rows.as<int>(), t.t_vals.as<float>());
+    ST_HIP(hipGetLastError());
+
+    t.N = (int)N; t.M = n_model; t.annz = annz; t.n_model = n_model; t.order = order;
+    t.s.reserve(sizeof(float) * N);
+    t.q.reserve(sizeof(float) * N);
+    t.r.reserve(sizeof(float) * n_model);
+    t.p.reserve(sizeof(float) * n_model);
+    t.x.reserve(sizeof(float) * n_model);
+    t.dots.reserve(sizeof(double) * (kDotBlocks + 8));
+    t.scal.reserve(sizeof(float) * 8);
+    double *part = t.dots.as<double>();
+    double *qTq = part + kDotBlocks, *rTr = qTq + 1, *rd = qTq + 2;
+    float *alpha = t.scal.as<float>(), *beta = alpha + 1;
+
+    const long long *rp = t.rowptr.as<long long>();
+    const long long *cp = t.t_colptr.as<long long>();
+    auto spmv = [&](const float *in, float *out) {  // out = A in
+        hipLaunchKernelGGL(spmv_csr, dim3((unsigned)((N * 16 + 255) / 256)), dim3(256), 0,
+                           ctx->stream, rp, t.cols.as<int>(), t.vals.as<float>(), in, out, (int)N);
+    };
+    auto spmvt = [&](const float *in, float *out) {  // out = A^T in
+        hipLaunchKernelGGL(spmv_csr, dim3((unsigned)(((long long)n_model * 16 + 255) / 256)),
+                           dim3(256), 0, ctx->stream, cp, t.t_rows.as<int>(), t.t_vals.as<float>(),
+                           in, out, n_model);
+    };
+    float *s = t.s.as<float>(), *q = t.q.as<float>(), *r = t.r.as<float>(), *p = t.p.as<float>(),
+          *x = t.x.as<float>();
+    // s = B (copied in by the caller into t.b), x = 0
+    ST_HIP(hipMemcpyAsync(s, t.b.ptr, sizeof(float) * N, hipMemcpyDeviceToDevice, ctx->stream));
+    ST_HIP(hipMemsetAsync(x, 0, sizeof(float) * n_model, ctx->stream));
+    spmvt(s, r);
+    ST_HIP(hipMemcpyAsync(p, r, sizeof(float) * n_model, hipMemcpyDeviceToDevice, ctx->stream));
+    spmv(p, q);
+    for (int it = 0; it < iters; it++) {
+        dot(ctx, q, q, N, part, qTq);
+        dot(ctx, r, r, n_model, part, rTr);
+        hipLaunchKernelGGL(ratio, dim3(1), dim3(64), 0, ctx->stream, rTr, qTq, alpha);
+        hipLaunchKernelGGL(axpy, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, x, p, alpha,
+                           1.0f, (long long)n_model);
+        hipLaunchKernelGGL(axpy, dim3(nblk256(N)), dim3(256), 0, ctx->stream, s, q, alpha, -1.0f, N);
+        dot(ctx, r, r, n_model, part, rd);
+        spmvt(s, r);
+        dot(ctx, r, r, n_model, part, rTr);
+        hipLaunchKernelGGL(ratio, dim3(1), dim3(64), 0, ctx->stream, rTr, rd, beta);
+        hipLaunchKernelGGL(xpby, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, p, r, beta,
+                           (long long)n_model);
+        spmv(p, q);
+    }
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    t.have_x = true;
+}
+
+// ---------------------------------------------------------------------------
+// model update
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float trilin(float c000, float c001, float c100, float c101, float c010,
+                                        float c011, float c110, float c111, float x0, float x1,
+                                        float y0, float y1, float z0, float z1, float x, float y,
+                                        float z) {
+    const float xd = (x - x0) / (x1 - x0);
+    const float yd = (y - y0) / (y1 - y0);
+    const float zd = (z - z0) / (z1 - z0);
+    const float c00 = c000 * (1 - xd) + c100 * xd;
+    const float c01 = c001 * (1 - xd) + c101 * xd;
+    const float c10 = c010 * (1 - xd) + c110 * xd;
+    const float c11 = c011 * (1 - xd) + c111 * xd;
+    const float c0 = c00 * (1 - yd) + c10 * yd;
+    const float c1 = c01 * (1 - yd) + c11 * yd;
+    return c0 * (1 - zd) + c1 * zd;
+}
+
+// dm on the model grid by trilinear interpolation of xdm (:500-541)
+__global__ void dm_trilinear(const float *__restrict__ xdm, float *__restrict__ dm, int nx, int ny,
+                             int nz, float dx, float dy, float dz, int nxt, int nyt, int nzt,
+                             float dxt, float dyt, float dzt) {
+    const long long index = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= (long long)nx * ny * nz) return;
+    const int k = (int)(index / ((long long)nx * nz));
+    const int j = (int)(index - (long long)k * nx * nz) / nz;
+    const int i = (int)(index - (long long)j * nz - (long long)k * nx * nz);
+    const float x = j * dx, y = k * dy, z = i * dz;
+    const float x0 = floorf(x / dxt) * dxt, y0 = floorf(y / dyt) * dyt, z0 = floorf(z / dzt) * dzt;
+    const float x1 = floorf(x / dxt) * dxt + dxt, y1 = floorf(y / dyt) * dyt + dyt,
+                z1 = floorf(z / dzt) * dzt + dzt;
+    float v = 0.0f;
+    if ((i >= 11) && (i < nz - 2) && (j >= 15) && (j < nx - 16) && (k >= 15) && (k < ny - 16)) {
+        const int idz = (int)(z / dzt), idx = (int)(x / dxt), idy = (int)(y / dyt);
+        const long long oz = (idz + 1 < nzt) ? 1 : 0;
+        const long long ox = (idx + 1 < nxt) ? nzt : 0;
+        const long long oy = (idy + 1 < nyt) ? (long long)nxt * nzt : 0;
+        const long long m = idz + (long long)idx * nzt + (long long)idy * nxt * nzt;
+        v = trilin(xdm[m], xdm[m + oz], xdm[m + ox], xdm[m + oz + ox], xdm[m + oy], xdm[m + oz + oy],
+                   xdm[m + ox + oy], xdm[m + oz + ox + oy], x0, x1, y0, y1, z0, z1, x, y, z);
+    }
+    dm[index] = v;
+}
+
+// gaussian (gaussian.cpp:65-90): interior cells get the weighted sum in
+// the reference's (yk, xk, zk) order, edge cells keep the input
+__global__ void gaussian3d(const float *__restrict__ in, float *__restrict__ out,
+                           const float *__restrict__ kern, int nx, int ny, int nz, int samples) {
+    const long long index = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= (long long)nx * ny * nz) return;
+    const int k = (int)(index / ((long long)nx * nz));
+    const int j = (int)(index - (long long)k * nx * nz) / nz;
+    const int i = (int)(index - (long long)j * nz - (long long)k * nx * nz);
+    const int init = samples / 2;
+    if (k < init || k >= ny - init || j < init || j >= nx - init || i < init || i >= nz - init) {
+        out[index] = in[index];
+        return;
+    }
+    float accum = 0.0f;
+    for (int yk = 0; yk < samples; yk++)
+        for (int xk = 0; xk < samples; xk++)
+            for (int zk = 0; zk < samples; zk++) {
+                const int w = zk + xk * samples + yk * samples * samples;
+                const long long src = (i - init + zk) + (long long)(j - init + xk) * nz +
+                                      (long long)(k - init + yk) * nx * nz;
+                accum += in[src] * kern[w];
+            }
+    out[index] = accum;
+}
+
+// slowness += dm inside the margins (:553-565), velocity = 1 / slowness
+__global__ void apply_update(float *__restrict__ slow, const float *__restrict__ dm,
+                             float *__restrict__ vel, int nx, int ny, int nz) {
+    const long long index = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= (long long)nx * ny * nz) return;
+    const int k = (int)(index / ((long long)nx * nz));
+    const int j = (int)(index - (long long)k * nx * nz) / nz;
+    const int i = (int)(index - (long long)j * nz - (long long)k * nx * nz);
+    float s = slow[index];
+    if ((i >= 11) && (i < nz - 2) && (j >= 15) && (j < nx - 15) && (k >= 15) && (k < ny - 15)) {
+        s += dm[index];
+        slow[index] = s;
+    }
+    vel[index] = 1.0f / s;
+}
+
+// Gaussian kernel weights as gaussian.cpp:7-43 builds them (host, FP32).
+static void gaussian_weights(int samples, float stdv, std::vector<float> &kernel) {
+    const int init = samples / 2;
+    const int nK = samples * samples * samples;
+    const float pi = 4.0f * atanf(1.0f);
+    kernel.assign(nK, 0.0f);
+    const int mid = samples / 2;
+    kernel[mid + mid * samples + mid * samples * samples] = 1.0f;
+    if (stdv != 0.0f) {
+        float sum = 0.0f;
+        for (int y = -init; y <= init; y++)
+            for (int x = -init; x <= init; x++)
+                for (int z = -init; z <= init; z++) {
+                    const int index = (z + init) + (x + init) * samples + (y + init) * samples * samples;
+                    kernel[index] = 1.0f / (2.0f * pi * stdv * stdv) *
+                                    expf(-((x * x + y * y + z * z) / (2.0f * stdv * stdv)));
+                    sum += kernel[index];
+                }
+        for (int i = 0; i < nK; i++) kernel[i] /= sum;
+    }
+}
+
+void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, float stdv,
+                       float *dm_out, float *vel_out) {
+    TomoState &t = tomo(ctx);
+    const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
+    t.dm.reserve(sizeof(float) * n);
+    t.upd.reserve(sizeof(float) * n);
+    hipLaunchKernelGGL(dm_trilinear, dim3(nblk256(n)), dim3(256), 0, ctx->stream, dxdm,
+                       t.dm.as<float>(), ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz,
+                       t.nxt, t.nyt, t.nzt, t.dxt, t.dyt, t.dzt);
+    float *dm = t.dm.as<float>();
+    if (smooth) {
+        ST_REQUIRE(samples >= 1 && (samples & 1), ST_ERR_ARG, "filter_samples must be odd");
+        std::vector<float> kern;
+        gaussian_weights(samples, stdv, kern);
+        t.kern.reserve(sizeof(float) * kern.size());
+        ST_HIP(hipMemcpyAsync(t.kern.ptr, kern.data(), sizeof(float) * kern.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(gaussian3d, dim3(nblk256(n)), dim3(256), 0, ctx->stream, dm,
+                           t.upd.as<float>(), t.kern.as<float>(), ctx->nx, ctx->ny, ctx->nz, samples);
+        ST_HIP(hipStreamSynchronize(ctx->stream));  // kern is reused; keep host vector alive
+        std::swap(t.dm, t.upd);
+        dm = t.dm.as<float>();
+    }
+    hipLaunchKernelGGL(apply_update, dim3(nblk256(n)), dim3(256), 0, ctx->stream,
+                       ctx->slow_zxy.as<float>(), dm, t.upd.as<float>(), ctx->nx, ctx->ny, ctx->nz);
+    ST_HIP(hipGetLastError());
+    if (dm_out)
+        ST_HIP(hipMemcpyAsync(dm_out, dm, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (vel_out)
+        ST_HIP(hipMemcpyAsync(vel_out, t.upd.ptr, sizeof(float) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->blk_valid[0] = ctx->blk_valid[1] = false;  // block layouts re-packed on next solve
+}
+
+void tomo_trace_zxy(st_ctx *ctx, const float *dT, const float *dsrc, const float *drcv, int nr,
+                    bool with_illum) {
+    trace_append(ctx, dT, (long long)ctx->nx * ctx->ny * ctx->nz, ZxyLay{ctx->nx, ctx->nz}, dsrc,
+                 1, drcv, nr, with_illum);
+}
+
+// ---------------------------------------------------------------------------
+// state helpers used by the C ABI
+// ---------------------------------------------------------------------------
+bool tomo_grid(st_ctx *ctx, int *n, float *d, int *max_steps, int *max_runs) {
+    if (!ctx->tomo) return false;
+    TomoState &t = tomo(ctx);
+    if (!t.grid_set) return false;
+    n[0] = t.nxt; n[1] = t.nyt; n[2] = t.nzt;
+    d[0] = t.dxt; d[1] = t.dyt; d[2] = t.dzt;
+    *max_steps = t.max_steps;
+    *max_runs = t.max_runs;
+    return true;
+}
+
+void tomo_set_grid(st_ctx *ctx, const int *n, const float *d, int max_steps, int max_runs) {
+    TomoState &t = tomo(ctx);
+    t.nxt = n[0]; t.nyt = n[1]; t.nzt = n[2];
+    t.dxt = d[0]; t.dyt = d[1]; t.dzt = d[2];
+    t.max_steps = max_steps;
+    t.max_runs = max_runs;
+    t.grid_set = true;
+    t.have_x = false;
+}
+
+void tomo_reset(st_ctx *ctx, bool with_illum) {
+    TomoState &t = tomo(ctx);
+    t.rows = 0;
+    t.nnz = 0;
+    t.have_x = false;
+    t.rowptr.reserve(sizeof(long long) * 1024);
+    ST_HIP(hipMemsetAsync(t.rowptr.ptr, 0, sizeof(long long), ctx->stream));
+    if (with_illum) {
+        const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
+        t.illum.reserve(sizeof(float) * n);
+        ST_HIP(hipMemsetAsync(t.illum.ptr, 0, sizeof(float) * n, ctx->stream));
+        t.illum_valid = true;
+    }
+}
+
+void tomo_get_illum(st_ctx *ctx, float *out) {
+    TomoState &t = tomo(ctx);
+    ST_REQUIRE(t.illum_valid, ST_ERR_STATE, "no illumination");
+    const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
+    ST_HIP(hipMemcpyAsync(out, t.illum.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+}
+
+void tomo_csr(st_ctx *ctx, long long *rows, long long *nnz, const long long **rp, const int **c,
+              const float **v) {
+    TomoState &t = tomo(ctx);
+    *rows = t.rows;
+    *nnz = t.nnz;
+    *rp = t.rowptr.as<long long>();
+    *c = t.cols.as<int>();
+    *v = t.vals.as<float>();
+}
+
+void tomo_load_csr(st_ctx *ctx, const long long *rowptr, const int *cols, const float *vals, int N,
+                   long long nnz) {
+    TomoState &t = tomo(ctx);
+    t.rowptr.reserve(sizeof(long long) * (N + 1));
+    t.cols.reserve(sizeof(int) * std::max(1LL, nnz));
+    t.vals.reserve(sizeof(float) * std::max(1LL, nnz));
+    ST_HIP(hipMemcpyAsync(t.rowptr.ptr, rowptr, sizeof(long long) * (N + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (nnz) {
+        ST_HIP(hipMemcpyAsync(t.cols.ptr, cols, sizeof(int) * nnz, hipMemcpyHostToDevice, ctx->stream));
+        ST_HIP(hipMemcpyAsync(t.vals.ptr, vals, sizeof(float) * nnz, hipMemcpyHostToDevice, ctx->stream));
+    }
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    t.rows = N;
+    t.nnz = nnz;
+    t.have_x = false;
+}
+
+// t.b = [dobs - dcal, 0 ...] of length Ntot (dcal nullable)
+void tomo_set_rhs(st_ctx *ctx, const float *dobs, const float *dcal, int n, long long Ntot) {
+    TomoState &t = tomo(ctx);
+    t.b.reserve(sizeof(float) * Ntot);
+    t.upd.reserve(sizeof(float) * 2 * n);
+    float *d0 = t.upd.as<float>(), *d1 = d0 + n;
+    ST_HIP(hipMemcpyAsync(d0, dobs, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (dcal)
+        ST_HIP(hipMemcpyAsync(d1, dcal, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    else
+        ST_HIP(hipMemsetAsync(d1, 0, sizeof(float) * n, ctx->stream));
+    hipLaunchKernelGGL(sub_vec, dim3(nblk256(Ntot)), dim3(256), 0, ctx->stream, d0, d1,
+                       t.b.as<float>(), (long long)n, Ntot);
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+const float *tomo_x(st_ctx *ctx) {
+    if (!ctx->tomo) return nullptr;
+    TomoState &t = tomo(ctx);
+    return t.have_x ? t.x.as<float>() : nullptr;
+}
+
+double tomo_residual(st_ctx *ctx, const float *dobs, const float *dcal, int n) {
+    TomoState &t = tomo(ctx);
+    t.upd.reserve(sizeof(float) * 3 * n);
+    t.dots.reserve(sizeof(double) * (kDotBlocks + 8));
+    float *d0 = t.upd.as<float>(), *d1 = d0 + n, *df = d1 + n;
+    ST_HIP(hipMemcpyAsync(d0, dobs, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    ST_HIP(hipMemcpyAsync(d1, dcal, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(sub_vec, dim3(nblk256(n)), dim3(256), 0, ctx->stream, d0, d1, df,
+                       (long long)n, (long long)n);
+    double *part = t.dots.as<double>();
+    dot(ctx, df, df, n, part, part + kDotBlocks);
+    double *h = static_cast<double *>(ctx->host_pinned) + 400;
+    ST_HIP(hipMemcpyAsync(h, part + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    return std::sqrt(*h);
+}
+
 }  // namespace st
