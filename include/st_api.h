@@ -44,6 +44,14 @@ typedef struct {
     int grid;           /* workgroups per round launch (0 = auto)              */
     int rounds_per_sync;/* rounds enqueued between host checks (0 = auto)      */
     int use_graph;      /* 1: replay rounds from a captured hipGraph           */
+    float band;         /* warm starts: visit a block only once its predicted
+                           arrival is within `band` s of its shot's earliest
+                           scheduled block (0 = default, <0 = off)             */
+    float cold_radius;  /* warm starts: cells closer than this to the source
+                           start cold (m, 0 = default)                         */
+    int warm_values;    /* warm starts: 1 initialise T = (1+delta)*prediction,
+                           0 keep T cold and use the prediction only to order
+                           block visits                                        */
 } st_fim_opts;
 
 typedef struct {
@@ -52,6 +60,8 @@ typedef struct {
     long long inner_iters;  /* inner iterations summed over visits        */
     long long fallbacks;    /* warm-started shots re-solved cold          */
     long long batches;      /* persistent solves launched                 */
+    long long deferred;     /* block visits postponed by arrival ordering */
+    double judge_max;       /* largest warm-start residual seen (s)       */
     double solve_ms;        /* device time of the FIM rounds              */
     double total_ms;        /* wall time of the call                      */
 } st_fim_stats;
@@ -85,6 +95,18 @@ int st_get_model(st_ctx *ctx, float *slowness);
 int st_fim_solve(st_ctx *ctx, const float *src_xyz, int nshot,
                  const float *rcv_xyz, int nrcv, float *picks, float *T_out,
                  const st_fim_opts *opts, st_fim_stats *stats);
+
+/* As st_fim_solve, warm-started from caller predictions T_pred (nshot
+ * volumes, z fastest; e.g. the previous tomography iteration's times).
+ * Each shot starts from (1+delta)*T_pred (cold near the source), block
+ * visits are ordered by predicted arrival, and the judge re-solves from a
+ * cold start every shot whose result is not a fixed point of the upwind
+ * update (a prediction that was not an upper bound).  No counterpart in the
+ * reference (its FIM always starts cold, block_FIM.cpp:295-369). */
+int st_fim_solve_warm(st_ctx *ctx, const float *src_xyz, int nshot,
+                      const float *rcv_xyz, int nrcv, float *picks, float *T_out,
+                      const float *T_pred, const st_fim_opts *opts,
+                      st_fim_stats *stats);
 
 /* Receiver picks on a caller volume: write_first_arrival's trilinear
  * (block_FIM.cpp:228-262, trilinear.cpp:3-20) on the device. */

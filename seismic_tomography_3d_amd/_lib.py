@@ -28,13 +28,16 @@ class FimOpts(ctypes.Structure):
                 ("max_inner", ctypes.c_int), ("eps", ctypes.c_float),
                 ("warm", ctypes.c_int), ("delta", ctypes.c_float),
                 ("judge", ctypes.c_int), ("grid", ctypes.c_int),
-                ("rounds_per_sync", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("rounds_per_sync", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("band", ctypes.c_float), ("cold_radius", ctypes.c_float),
+                ("warm_values", ctypes.c_int)]
 
 
 class FimStats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_longlong), ("visits", ctypes.c_longlong),
                 ("inner_iters", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong),
-                ("batches", ctypes.c_longlong), ("solve_ms", ctypes.c_double),
+                ("batches", ctypes.c_longlong), ("deferred", ctypes.c_longlong),
+                ("judge_max", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -53,6 +56,9 @@ SIGNATURES = [
     ("st_fim_solve", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
                                     ctypes.c_int, c_float_p, c_float_p,
                                     ctypes.POINTER(FimOpts), ctypes.POINTER(FimStats)]),
+    ("st_fim_solve_warm", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                         ctypes.c_int, c_float_p, c_float_p, c_float_p,
+                                         ctypes.POINTER(FimOpts), ctypes.POINTER(FimStats)]),
     ("st_picks_volume", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, c_float_p, ctypes.c_int, c_float_p]),

@@ -100,6 +100,21 @@ class Context:
                                           ctypes.byref(o), ctypes.byref(st)))
         return picks, T, st.as_dict()
 
+    def fim_solve_warm(self, src, rcv, T_pred, return_T=False, **opts):
+        """Warm-started solve from caller predictions T_pred[nshot, ny, nx, nz]."""
+        src = _f32(src, (-1, 3))
+        rcv = _f32(rcv, (-1, 3)) if rcv is not None else np.zeros((0, 3), np.float32)
+        ns, nr = src.shape[0], rcv.shape[0]
+        Tp = _f32(T_pred, (ns,) + self.shape)
+        picks = np.empty((ns, nr), np.float32)
+        T = np.empty((ns,) + self.shape, np.float32) if return_T else None
+        o = self.options(**opts)
+        st = FimStats()
+        self._check(self.lib.st_fim_solve_warm(self.h, _ptr(src), ns, _ptr(rcv) if nr else None,
+                                               nr, _ptr(picks) if nr else None, _ptr(T), _ptr(Tp),
+                                               ctypes.byref(o), ctypes.byref(st)))
+        return picks, T, st.as_dict()
+
     def picks_volume(self, T, spacing, rcv):
         T = _f32(T)
         ny, nx, nz = T.shape

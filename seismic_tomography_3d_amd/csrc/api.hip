@@ -9,7 +9,10 @@
 namespace st {
 void ensure_block_model(st_ctx *ctx, int B);
 void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
-               st_fim_stats *stats, bool warm_loaded);
+               st_fim_stats *stats, bool warm_loaded, const float *keys);
+void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nshot, int B,
+                    float delta, float r_cold, bool values, float *dkeys);
+void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res);
 void launch_extract(st_ctx *ctx, float *dout, int nshot, int B);
 void launch_picks_blk(hipStream_t st, const float *T, long long vol, int nshot, const float *rcv,
                       int nr, float *out, int nx, int ny, int nz, float dx, float dy, float dz,
@@ -17,8 +20,9 @@ void launch_picks_blk(hipStream_t st, const float *T, long long vol, int nshot, 
 void launch_picks_zxy(hipStream_t st, const float *T, long long vol, int nshot, const float *rcv,
                       int nr, float *out, int nx, int ny, int nz, float dx, float dy, float dz);
 // surrogate warm start (surrogate.hip); returns false when no network
-bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float delta);
-int surrogate_judge(st_ctx *ctx, const float *dsrc, int nshot, int B, float tol, int *bad);
+bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float delta,
+                         float r_cold, bool values, float *dkeys);
+float surrogate_cold_radius(st_ctx *ctx);
 void tomo_release(st_ctx *ctx);
 void surrogate_release(st_ctx *ctx);
 struct TomoState;
@@ -88,6 +92,9 @@ void st_fim_default_opts(st_fim_opts *o) {
     o->grid = 0;
     o->rounds_per_sync = 0;
     o->use_graph = 0;
+    o->band = 0.0f;
+    o->cold_radius = 0.0f;
+    o->warm_values = 1;
 }
 
 int st_ctx_create(int device, st_ctx **out) {
@@ -124,7 +131,7 @@ int st_ctx_destroy(st_ctx *ctx) {
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (auto *b : {&ctx->slow_zxy, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
                     &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->src, &ctx->rcv,
-                    &ctx->picks, &ctx->scratch})
+                    &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred})
         b->release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -155,6 +162,8 @@ int st_set_model(st_ctx *ctx, const float *slowness, int nx, int ny, int nz, flo
                               hipMemcpyHostToDevice, ctx->stream));
         ctx->blk_valid[0] = ctx->blk_valid[1] = false;
         ctx->has_model = true;
+        ctx->smin = *std::min_element(slowness, slowness + n);
+        ctx->smax = *std::max_element(slowness, slowness + n);
         ST_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -172,96 +181,161 @@ int st_get_model(st_ctx *ctx, float *slowness) {
     });
 }
 
+static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
+                       int nrcv, float *picks, float *T_out, const float *T_pred,
+                       const st_fim_opts *opts, st_fim_stats *stats) {
+    const auto t_start = std::chrono::steady_clock::now();
+    ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+    ST_REQUIRE(nshot >= 0 && nrcv >= 0, ST_ERR_ARG, "negative count");
+    ST_REQUIRE(nshot == 0 || src_xyz, ST_ERR_ARG, "null sources");
+    ST_REQUIRE(nrcv == 0 || (rcv_xyz && picks), ST_ERR_ARG, "null receivers or picks");
+    st_fim_opts o;
+    if (opts) o = *opts; else st_fim_default_opts(&o);
+    ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
+    if (o.max_inner <= 0) o.max_inner = 16;
+    if (o.eps < 0.f) o.eps = 0.f;
+    if (o.delta <= 0.f) o.delta = 0.02f;
+    ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
+    const bool warm = (o.warm == 1) || T_pred;
+    if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
+    if (o.band < 0.f || !warm) o.band = 1e30f;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (nshot == 0) return;
+    check_points(src_xyz, nshot, ctx, "source");
+    check_points(rcv_xyz, nrcv, ctx, "receiver");
+    set_device(ctx);
+
+    const int B = o.block;
+    ensure_block_model(ctx, B);
+    const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+    ST_REQUIRE(g.nblk < (1LL << 22), ST_ERR_ARG, "model has too many blocks");
+    const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+    int batch = std::max(1, std::min({o.batch, nshot, 1023}));
+    size_t free_b = 0, total_b = 0;
+    ST_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_shot = sizeof(float) * g.vol + 3 * sizeof(unsigned) * g.nblk +
+                            (warm ? sizeof(float) * g.nblk : 0) +
+                            (T_pred ? sizeof(float) * npts : 0) + (T_out ? sizeof(float) * npts : 0);
+    const size_t have = free_b + ctx->T.bytes + ctx->flags.bytes + ctx->list[0].bytes +
+                        ctx->list[1].bytes + ctx->keys.bytes + ctx->pred.bytes + ctx->scratch.bytes;
+    while (batch > 1 && per_shot * (size_t)batch > have / 2) batch /= 2;
+
+    ctx->T.reserve(sizeof(float) * g.vol * batch);
+    ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
+    ctx->list[0].reserve(sizeof(unsigned) * g.nblk * batch);
+    ctx->list[1].reserve(sizeof(unsigned) * g.nblk * batch);
+    ctx->src.reserve(sizeof(float) * 3 * (nshot + batch));
+    ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
+    ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
+    if (warm) ctx->keys.reserve(sizeof(float) * g.nblk * batch);
+    if (T_pred) ctx->pred.reserve(sizeof(float) * npts * batch);
+    if (T_out) ctx->scratch.reserve(sizeof(float) * npts * batch);
+    ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot, hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (nrcv)
+        ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
+                              hipMemcpyHostToDevice, ctx->stream));
+    float r_cold = o.cold_radius;
+    if (r_cold <= 0.f)
+        r_cold = (o.warm == 1) ? surrogate_cold_radius(ctx) : 2.0f * B * ctx->dx;
+
+    std::vector<float> res(batch);
+    for (int b0 = 0; b0 < nshot; b0 += batch) {
+        const int nb = std::min(batch, nshot - b0);
+        const float *dsrc = ctx->src.as<float>() + 3LL * b0;
+        const float *keys = nullptr;
+        if (warm) {
+            if (T_pred) {
+                ST_HIP(hipMemcpyAsync(ctx->pred.ptr, T_pred + (long long)b0 * npts,
+                                      sizeof(float) * npts * nb, hipMemcpyHostToDevice, ctx->stream));
+                warm_fill_fine(ctx, ctx->pred.as<float>(), dsrc, nb, B, o.delta, r_cold,
+                               o.warm_values != 0, ctx->keys.as<float>());
+            } else {
+                const bool ok = surrogate_warm_fill(ctx, dsrc, nb, B, o.delta, r_cold,
+                                                    o.warm_values != 0, ctx->keys.as<float>());
+                ST_REQUIRE(ok, ST_ERR_NO_SURROGATE, "warm start requested without a network");
+            }
+            keys = ctx->keys.as<float>();
+        }
+        fim_batch(ctx, dsrc, nb, o, stats, warm, keys);
+        std::vector<int> bad;
+        if (warm && o.judge) {
+            judge_batch(ctx, dsrc, nb, B, res.data());
+            for (int j = 0; j < nb; j++) {
+                if (stats) stats->judge_max = std::max(stats->judge_max, (double)res[j]);
+                if (res[j] > 2e-6f) bad.push_back(j);
+            }
+        }
+        if (nrcv)
+            launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, ctx->rcv.as<float>(), nrcv,
+                             ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx, ctx->ny,
+                             ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
+        if (T_out) {
+            launch_extract(ctx, ctx->scratch.as<float>(), nb, B);
+            ST_HIP(hipMemcpyAsync(T_out + (long long)b0 * npts, ctx->scratch.ptr,
+                                  sizeof(float) * npts * nb, hipMemcpyDeviceToHost, ctx->stream));
+            ST_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (!bad.empty()) {
+            // fallback: the rejected shots are solved again from a cold start
+            const int k = (int)bad.size();
+            float *gsrc = ctx->src.as<float>() + 3LL * nshot;
+            std::vector<float> hs(3 * k);
+            for (int q = 0; q < k; q++)
+                for (int c = 0; c < 3; c++) hs[3 * q + c] = src_xyz[3LL * (b0 + bad[q]) + c];
+            ST_HIP(hipMemcpyAsync(gsrc, hs.data(), sizeof(float) * 3 * k, hipMemcpyHostToDevice,
+                                  ctx->stream));
+            st_fim_opts oc = o;
+            oc.warm = 0;
+            oc.band = 1e30f;
+            fim_batch(ctx, gsrc, k, oc, stats, false, nullptr);
+            if (stats) stats->fallbacks += k;
+            for (int q = 0; q < k; q++) {
+                const long long row = b0 + bad[q];
+                if (nrcv)
+                    launch_picks_blk(ctx->stream, ctx->T.as<float>() + q * g.vol, g.vol, 1,
+                                     ctx->rcv.as<float>(), nrcv,
+                                     ctx->picks.as<float>() + row * nrcv, ctx->nx, ctx->ny,
+                                     ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
+            }
+            if (T_out) {
+                launch_extract(ctx, ctx->scratch.as<float>(), k, B);
+                for (int q = 0; q < k; q++)
+                    ST_HIP(hipMemcpyAsync(T_out + (long long)(b0 + bad[q]) * npts,
+                                          ctx->scratch.as<float>() + q * npts, sizeof(float) * npts,
+                                          hipMemcpyDeviceToHost, ctx->stream));
+                ST_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        }
+    }
+    if (nrcv)
+        ST_HIP(hipMemcpyAsync(picks, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+                .count();
+}
+
 int st_fim_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz, int nrcv,
                  float *picks, float *T_out, const st_fim_opts *opts, st_fim_stats *stats) {
     if (!ctx) return ST_ERR_ARG;
     return guarded(ctx, [&] {
-        const auto t_start = std::chrono::steady_clock::now();
-        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
-        ST_REQUIRE(nshot >= 0 && nrcv >= 0, ST_ERR_ARG, "negative count");
-        ST_REQUIRE(nshot == 0 || src_xyz, ST_ERR_ARG, "null sources");
-        ST_REQUIRE(nrcv == 0 || (rcv_xyz && picks), ST_ERR_ARG, "null receivers or picks");
+        solve_impl(ctx, src_xyz, nshot, rcv_xyz, nrcv, picks, T_out, nullptr, opts, stats);
+    });
+}
+
+int st_fim_solve_warm(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
+                      int nrcv, float *picks, float *T_out, const float *T_pred,
+                      const st_fim_opts *opts, st_fim_stats *stats) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(T_pred, ST_ERR_ARG, "null predictions");
         st_fim_opts o;
         if (opts) o = *opts; else st_fim_default_opts(&o);
-        ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
-        if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 8 : 16;
-        if (o.eps < 0.f) o.eps = 0.f;
-        if (o.delta <= 0.f) o.delta = 0.02f;
-        ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
-        if (stats) std::memset(stats, 0, sizeof(*stats));
-        if (nshot == 0) return;
-        check_points(src_xyz, nshot, ctx, "source");
-        check_points(rcv_xyz, nrcv, ctx, "receiver");
-        set_device(ctx);
-
-        const int B = o.block;
-        ensure_block_model(ctx, B);
-        const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
-        ST_REQUIRE(g.nblk < (1LL << 22), ST_ERR_ARG, "model has too many blocks");
-        int batch = std::max(1, std::min({o.batch, nshot, 1023}));
-        // keep the batch's travel times within a quarter of free HBM
-        size_t free_b = 0, total_b = 0;
-        ST_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t per_shot = sizeof(float) * g.vol + 3 * sizeof(unsigned) * g.nblk;
-        const size_t have = free_b + ctx->T.bytes + ctx->flags.bytes + ctx->list[0].bytes +
-                            ctx->list[1].bytes;
-        while (batch > 1 && per_shot * (size_t)batch > have / 2) batch /= 2;
-
-        ctx->T.reserve(sizeof(float) * g.vol * batch);
-        ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
-        ctx->list[0].reserve(sizeof(unsigned) * g.nblk * batch);
-        ctx->list[1].reserve(sizeof(unsigned) * g.nblk * batch);
-        ctx->src.reserve(sizeof(float) * 3 * nshot);
-        ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
-        ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
-        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
-                              hipMemcpyHostToDevice, ctx->stream));
-        if (nrcv)
-            ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
-                                  hipMemcpyHostToDevice, ctx->stream));
-        const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
-        if (T_out) ctx->scratch.reserve(sizeof(float) * npts * batch);
-
-        for (int b0 = 0; b0 < nshot; b0 += batch) {
-            const int nb = std::min(batch, nshot - b0);
-            const float *dsrc = ctx->src.as<float>() + 3LL * b0;
-            bool warm = false;
-            if (o.warm == 1) {
-                warm = surrogate_warm_fill(ctx, dsrc, nb, B, o.delta);
-                ST_REQUIRE(warm, ST_ERR_NO_SURROGATE, "warm start requested without a network");
-            }
-            fim_batch(ctx, dsrc, nb, o, stats, warm);
-            if (warm && o.judge) {
-                // judge: a warm start that was not an upper bound leaves cells
-                // below what their neighbours support; such batches are
-                // re-solved from a cold start (the fallback)
-                int bad[1024];
-                const int nbad = surrogate_judge(ctx, dsrc, nb, B, 1e-5f, bad);
-                if (nbad > 0) {
-                    if (stats) stats->fallbacks += nbad;
-                    fim_batch(ctx, dsrc, nb, o, stats, false);
-                }
-            }
-            if (nrcv)
-                launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, ctx->rcv.as<float>(),
-                                 nrcv, ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx,
-                                 ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
-            if (T_out) {
-                launch_extract(ctx, ctx->scratch.as<float>(), nb, B);
-                ST_HIP(hipMemcpyAsync(T_out + (long long)b0 * npts, ctx->scratch.ptr,
-                                      sizeof(float) * npts * nb, hipMemcpyDeviceToHost,
-                                      ctx->stream));
-                ST_HIP(hipStreamSynchronize(ctx->stream));
-            }
-        }
-        if (nrcv)
-            ST_HIP(hipMemcpyAsync(picks, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
-                                  hipMemcpyDeviceToHost, ctx->stream));
-        ST_HIP(hipStreamSynchronize(ctx->stream));
-        if (stats)
-            stats->total_ms = std::chrono::duration<double, std::milli>(
-                                  std::chrono::steady_clock::now() - t_start)
-                                  .count();
+        o.warm = 0;  // the predictions come from the caller
+        solve_impl(ctx, src_xyz, nshot, rcv_xyz, nrcv, picks, T_out, T_pred, &o, stats);
     });
 }
 
@@ -368,7 +442,7 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
         for (int b0 = 0; b0 < nshot; b0 += batch) {
             const int nb = std::min(batch, nshot - b0);
             const float *dsrc = ctx->src.as<float>() + 3LL * b0;
-            fim_batch(ctx, dsrc, nb, o, stats, false);
+            fim_batch(ctx, dsrc, nb, o, stats, false, nullptr);
             launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, ctx->rcv.as<float>(), nrcv,
                              ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx, ctx->ny,
                              ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);

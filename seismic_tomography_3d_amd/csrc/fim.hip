@@ -43,7 +43,16 @@ struct FimArgs {
     float h, eps;
     int max_inner;
     int round;                 // round index modulo 6
+    // predicted-arrival scheduling (nullable keys = plain FIM): a block whose
+    // key exceeds (smallest key of its shot's current list + band) is
+    // deferred to the next round instead of visited
+    const float *keys;         // per (shot, block) predicted first arrival
+    unsigned *kmin;            // [3][nshot] per-shot smallest key of each list (float bits)
+    int nshot;
+    float band;
 };
+
+constexpr unsigned kInfBits = 0x7F800000u;
 
 constexpr int kShotShift = 22;
 constexpr unsigned kBlkMask = (1u << kShotShift) - 1u;
@@ -86,6 +95,9 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
     if (target != 0xFFFFFFFFu) {
         const unsigned old = atomicOr(&fl[target], bit_nxt);
         fresh = !(old & bit_nxt);
+        if (fresh && A.keys)
+            atomicMin(&A.kmin[((A.round + 1) % 3) * A.nshot + shot],
+                      __float_as_uint(A.keys[(long long)shot * A.nblk + target]));
     }
     const unsigned long long fm = __ballot(fresh);
     if (fm) {
@@ -107,9 +119,13 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
     const int lane = threadIdx.x;
     const int p = A.round & 1;
     const unsigned count = A.cnt[A.round % 3];
-    if (blockIdx.x == 0 && lane == 0) {
-        A.cnt[(A.round + 2) % 3] = 0;
-        if (count) atomicAdd(&A.cnt[3], 1u);
+    if (blockIdx.x == 0) {
+        if (lane == 0) {
+            A.cnt[(A.round + 2) % 3] = 0;
+            if (count) atomicAdd(&A.cnt[3], 1u);
+        }
+        if (A.keys)
+            for (int j = lane; j < A.nshot; j += 64) A.kmin[((A.round + 2) % 3) * A.nshot + j] = kInfBits;
     }
     if (count == 0) return;
     const unsigned *cur = p ? A.list1 : A.list0;
@@ -119,11 +135,8 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
 
     const int tx = lane & 3, ty = (lane >> 2) & 3, tz = lane >> 4;
     const int tc = (tx + 1) + 6 * (ty + 1) + 36 * (tz + 1);
-    constexpr unsigned long long kXm = 0x1111111111111111ull, kXp = 0x8888888888888888ull;
-    constexpr unsigned long long kYm = 0x000F000F000F000Full, kYp = 0xF000F000F000F000ull;
-    constexpr unsigned long long kZm = 0x000000000000FFFFull, kZp = 0xFFFF000000000000ull;
     const int nbxy = A.nbx * A.nby;
-    unsigned long long my_visits = 0, my_iters = 0;
+    unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
     for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
         const unsigned item = cur[i];
@@ -131,6 +144,14 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
         if (lane == 0) atomicAnd(&fl[b], ~bit_cur);
+        if (A.keys) {
+            const float tau = __uint_as_float(A.kmin[(A.round % 3) * A.nshot + shot]) + A.band;
+            if (A.keys[(long long)shot * A.nblk + b] > tau) {  // not its time yet
+                schedule(A, shot, fl, lane == 0 ? b : 0xFFFFFFFFu, bit_nxt, nxt, nxt_cnt, lane);
+                my_defer += 1;
+                continue;
+            }
+        }
         const int bx = b % A.nbx, rest = b / A.nbx, by = rest % A.nby, bz = rest / A.nby;
         const long long base = (long long)b * 64;
 
@@ -182,16 +203,26 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
 
         const float dl = v0 - v;
         if (__ballot(dl > 0.0f)) Tsh[base + lane] = v;
-        const unsigned long long m = __ballot(dl > A.eps);
+        // a changed face cell can lower the neighbour only if it is now
+        // below the neighbour's adjacent cell (held in the halo since the
+        // visit began; the neighbour has only decreased since, so the test
+        // never misses a needed visit)
+        const bool ch = dl > A.eps;
+        const bool xm = __ballot(ch && tx == 0 && v < tile[tc - 1]) != 0ull;
+        const bool xp = __ballot(ch && tx == 3 && v < tile[tc + 1]) != 0ull;
+        const bool ym = __ballot(ch && ty == 0 && v < tile[tc - 6]) != 0ull;
+        const bool yp = __ballot(ch && ty == 3 && v < tile[tc + 6]) != 0ull;
+        const bool zm = __ballot(ch && tz == 0 && v < tile[tc - 36]) != 0ull;
+        const bool zp = __ballot(ch && tz == 3 && v < tile[tc + 36]) != 0ull;
         unsigned target = 0xFFFFFFFFu;
         switch (lane) {
         case 0: if (more) target = b; break;
-        case 1: if ((m & kXm) && bx > 0) target = b - 1; break;
-        case 2: if ((m & kXp) && bx < A.nbx - 1) target = b + 1; break;
-        case 3: if ((m & kYm) && by > 0) target = b - A.nbx; break;
-        case 4: if ((m & kYp) && by < A.nby - 1) target = b + A.nbx; break;
-        case 5: if ((m & kZm) && bz > 0) target = b - nbxy; break;
-        case 6: if ((m & kZp) && bz < A.nbz - 1) target = b + nbxy; break;
+        case 1: if (xm && bx > 0) target = b - 1; break;
+        case 2: if (xp && bx < A.nbx - 1) target = b + 1; break;
+        case 3: if (ym && by > 0) target = b - A.nbx; break;
+        case 4: if (yp && by < A.nby - 1) target = b + A.nbx; break;
+        case 5: if (zm && bz > 0) target = b - nbxy; break;
+        case 6: if (zp && bz < A.nbz - 1) target = b + nbxy; break;
         default: break;
         }
         schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane);
@@ -202,6 +233,7 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
     if (lane == 0 && A.stats) {
         atomicAdd(&A.stats[0], my_visits);
         atomicAdd(&A.stats[1], my_iters);
+        if (my_defer) atomicAdd(&A.stats[2], my_defer);
     }
 }
 
@@ -215,9 +247,13 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
     const int lane = threadIdx.x;
     const int p = A.round & 1;
     const unsigned count = A.cnt[A.round % 3];
-    if (blockIdx.x == 0 && lane == 0) {
-        A.cnt[(A.round + 2) % 3] = 0;
-        if (count) atomicAdd(&A.cnt[3], 1u);
+    if (blockIdx.x == 0) {
+        if (lane == 0) {
+            A.cnt[(A.round + 2) % 3] = 0;
+            if (count) atomicAdd(&A.cnt[3], 1u);
+        }
+        if (A.keys)
+            for (int j = lane; j < A.nshot; j += 64) A.kmin[((A.round + 2) % 3) * A.nshot + j] = kInfBits;
     }
     if (count == 0) return;
     const unsigned *cur = p ? A.list1 : A.list0;
@@ -228,7 +264,7 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
     const int tx = lane & 7, ty = lane >> 3;
     const int tcol = (tx + 1) + 10 * (ty + 1);  // tile index of (tx, ty, z=-1)+100
     const int nbxy = A.nbx * A.nby;
-    unsigned long long my_visits = 0, my_iters = 0;
+    unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
     for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
         const unsigned item = cur[i];
@@ -236,6 +272,14 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
         if (lane == 0) atomicAnd(&fl[b], ~bit_cur);
+        if (A.keys) {
+            const float tau = __uint_as_float(A.kmin[(A.round % 3) * A.nshot + shot]) + A.band;
+            if (A.keys[(long long)shot * A.nblk + b] > tau) {  // not its time yet
+                schedule(A, shot, fl, lane == 0 ? b : 0xFFFFFFFFu, bit_nxt, nxt, nxt_cnt, lane);
+                my_defer += 1;
+                continue;
+            }
+        }
         const int bx = b % A.nbx, rest = b / A.nbx, by = rest % A.nby, bz = rest / A.nby;
         const long long base = (long long)b * 512;
         const bool vxy = (bx * 8 + tx < A.n4x) && (by * 8 + ty < A.n4y);
@@ -305,33 +349,38 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
         #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = vz[k + 1];
 
-        bool anydec = false, fz0 = false, fz7 = false, fall = false;
+        // face activation as in the 4^3 kernel: a changed face cell must be
+        // below the neighbour's adjacent cell (tile halo / z halo registers)
+        bool anydec = false, fxm = false, fxp = false, fym = false, fyp = false;
         #pragma unroll
         for (int k = 0; k < 8; k++) {
             const float dl = v0[k] - v[k];
             anydec |= dl > 0.0f;
             const bool c = dl > A.eps;
-            fall |= c;
-            if (k == 0) fz0 = c;
-            if (k == 7) fz7 = c;
+            const int ti = tcol + 100 * (k + 1);
+            fxm |= c && tx == 0 && v[k] < tile[ti - 1];
+            fxp |= c && tx == 7 && v[k] < tile[ti + 1];
+            fym |= c && ty == 0 && v[k] < tile[ti - 10];
+            fyp |= c && ty == 7 && v[k] < tile[ti + 10];
         }
+        const bool fzm = (v0[0] - v[0] > A.eps) && v[0] < vz[0];
+        const bool fzp = (v0[7] - v[7] > A.eps) && v[7] < vz[9];
         if (__ballot(anydec)) {
             #pragma unroll
             for (int k = 0; k < 8; k++) Tsh[base + lane + 64 * k] = v[k];
         }
-        const unsigned long long m = __ballot(fall);
-        const bool zm_ch = __ballot(fz0) != 0ull, zp_ch = __ballot(fz7) != 0ull;
-        constexpr unsigned long long kXm = 0x0101010101010101ull, kXp = 0x8080808080808080ull;
-        constexpr unsigned long long kYm = 0x00000000000000FFull, kYp = 0xFF00000000000000ull;
+        const bool xm = __ballot(fxm) != 0ull, xp = __ballot(fxp) != 0ull;
+        const bool ym = __ballot(fym) != 0ull, yp = __ballot(fyp) != 0ull;
+        const bool zm = __ballot(fzm) != 0ull, zp = __ballot(fzp) != 0ull;
         unsigned target = 0xFFFFFFFFu;
         switch (lane) {
         case 0: if (more) target = b; break;
-        case 1: if ((m & kXm) && bx > 0) target = b - 1; break;
-        case 2: if ((m & kXp) && bx < A.nbx - 1) target = b + 1; break;
-        case 3: if ((m & kYm) && by > 0) target = b - A.nbx; break;
-        case 4: if ((m & kYp) && by < A.nby - 1) target = b + A.nbx; break;
-        case 5: if (zm_ch && bz > 0) target = b - nbxy; break;
-        case 6: if (zp_ch && bz < A.nbz - 1) target = b + nbxy; break;
+        case 1: if (xm && bx > 0) target = b - 1; break;
+        case 2: if (xp && bx < A.nbx - 1) target = b + 1; break;
+        case 3: if (ym && by > 0) target = b - A.nbx; break;
+        case 4: if (yp && by < A.nby - 1) target = b + A.nbx; break;
+        case 5: if (zm && bz > 0) target = b - nbxy; break;
+        case 6: if (zp && bz < A.nbz - 1) target = b + nbxy; break;
         default: break;
         }
         schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane);
@@ -342,6 +391,7 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
     if (lane == 0 && A.stats) {
         atomicAdd(&A.stats[0], my_visits);
         atomicAdd(&A.stats[1], my_iters);
+        if (my_defer) atomicAdd(&A.stats[2], my_defer);
     }
 }
 
@@ -376,7 +426,7 @@ __global__ void fill_inf(float4 *__restrict__ T, long long n4) {
 __global__ void seed_sources(const float *__restrict__ src, int nshot, const float *__restrict__ S,
                              float *T, unsigned *flags, unsigned *list0, unsigned *cnt,
                              float dx, float dy, float dz, int B, int nbx, int nby,
-                             long long vol, long long nblk) {
+                             long long vol, long long nblk, const float *keys, unsigned *kmin) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) { cnt[0] = nshot; cnt[1] = 0; cnt[2] = 0; cnt[3] = 0; }
     if (j >= nshot) return;
@@ -388,6 +438,11 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
     const unsigned b = (unsigned)(a / (B * B * B));
     flags[(long long)j * nblk + b] = 1u;
     list0[j] = ((unsigned)j << kShotShift) | b;
+    if (keys) {
+        kmin[j] = __float_as_uint(keys[(long long)j * nblk + b]);
+        kmin[nshot + j] = kInfBits;
+        kmin[2 * nshot + j] = kInfBits;
+    }
 }
 
 // block layout -> z-fastest model extent (extract_solution + reduce_model,
@@ -404,6 +459,96 @@ __global__ void extract_volume(const float *__restrict__ T, float *__restrict__ 
     out[i] = T[sh * vol + blk_addr(x, y, z, B, nbx, nby)];
 }
 
+
+// ---------------------------------------------------------------------------
+// warm start: T0 = (1 + delta) * prediction, cold (kInf) near the source and
+// on the pad cells past the model; per-block key = smallest prediction in
+// the block (the block's predicted first arrival).  One wave per block.
+// ---------------------------------------------------------------------------
+struct FinePred {  // caller volumes, z fastest, one per shot
+    const float *P;
+    int nx, nz;
+    long long npts;
+    __device__ float operator()(int s, int x, int y, int z, float, float, float, float) const {
+        return P[s * npts + z + (long long)x * nz + (long long)y * nx * nz];
+    }
+};
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+
+template <class Pred>
+__global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__restrict__ keys,
+                                                const float *__restrict__ src, int B, int nbx,
+                                                int nby, long long nblk, long long vol, int nx,
+                                                int ny, int nz, float dx, float dy, float dz,
+                                                float scale, float r_cold, int write_values,
+                                                Pred pred) {
+    const long long w = blockIdx.x;
+    const int s = (int)(w / nblk);
+    const long long b = w - (long long)s * nblk;
+    const int lane = threadIdx.x;
+    const int B3 = B * B * B;
+    const int bx = (int)(b % nbx);
+    const long long rest = b / nbx;
+    const int by = (int)(rest % nby), bz = (int)(rest / nby);
+    const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+    float kmin = INFINITY;
+    for (int t = lane; t < B3; t += 64) {
+        const int tx = t % B, ty = (t / B) % B, tz = t / (B * B);
+        const int x = bx * B + tx, y = by * B + ty, z = bz * B + tz;
+        float v = kInf;
+        if (x < nx && y < ny && z < nz) {
+            const float px = x * dx, py = y * dy, pz = z * dz;
+            const float ex = px - sx, ey = py - sy, ez = pz - sz;
+            const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+            const float p = pred(s, x, y, z, px, py, pz, dist);
+            kmin = fminf(kmin, p);
+            if (dist >= r_cold) v = fminf(scale * p, kInf);
+        }
+        if (write_values) T[s * vol + b * B3 + t] = v;
+    }
+    kmin = wave_min(kmin);
+    if (lane == 0) keys[s * nblk + b] = isfinite(kmin) ? fmaxf(kmin, 0.0f) : 0.0f;
+}
+
+// Judge: largest residual G(T) - T over the domain cells of each shot
+// (source node excluded).  A converged solve from an upper bound has none:
+// every cell holds the update of its final neighbours.  A positive residual
+// marks cells whose warm value was below what their neighbours support.
+__global__ void judge_residual(const float *__restrict__ T, const float *__restrict__ S,
+                               const float *__restrict__ src, int nshot, int B, int nbx, int nby,
+                               int n4x, int n4y, int n4z, long long vol, float dx, float dy,
+                               float dz, unsigned *__restrict__ resmax) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nshot * vol) return;
+    const int s = (int)(i / vol);
+    const long long a = i - (long long)s * vol;
+    const int B3 = B * B * B;
+    const long long b = a / B3;
+    const int t = (int)(a - b * B3);
+    const int x = (int)(b % nbx) * B + t % B;
+    const long long rest = b / nbx;
+    const int y = (int)(rest % nby) * B + (t / B) % B;
+    const int z = (int)(rest / nby) * B + t / (B * B);
+    if (x >= n4x || y >= n4y || z >= n4z) return;
+    const int ix = (int)(src[3 * s] / dx), iy = (int)(src[3 * s + 1] / dy), iz = (int)(src[3 * s + 2] / dz);
+    if (x == ix && y == iy && z == iz) return;
+    const float *Ts = T + (long long)s * vol;
+    auto at = [&](int xx, int yy, int zz) -> float {
+        if (xx < 0 || yy < 0 || zz < 0 || xx >= n4x || yy >= n4y || zz >= n4z) return kInf;
+        return Ts[blk_addr(xx, yy, zz, B, nbx, nby)];
+    };
+    const float v = Ts[a];
+    const float hs = dx * S[a];
+    const float g = godunov(fminf(at(x - 1, y, z), at(x + 1, y, z)),
+                            fminf(at(x, y - 1, z), at(x, y + 1, z)),
+                            fminf(at(x, y, z - 1), at(x, y, z + 1)), hs, hs * hs);
+    const float r = g - v;
+    if (r > 0.0f) atomicMax(&resmax[s], __float_as_uint(r));
+}
 }  // namespace st
 
 // ---------------------------------------------------------------------------
@@ -431,13 +576,14 @@ const float *block_model(st_ctx *ctx, int B) { return ctx->slow_blk[blk_index(B)
 // Device-resident FIM over one batch of shots already uploaded to dsrc.
 // Leaves the travel times in ctx->T (shot-major block layout).
 void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
-               st_fim_stats *stats, bool warm_loaded) {
+               st_fim_stats *stats, bool warm_loaded, const float *keys) {
     const int B = o.block;
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     unsigned *cnt = ctx->counters.as<unsigned>();
     unsigned long long *dstats = reinterpret_cast<unsigned long long *>(cnt + 8);
     const int tpb = 256;
 
+    if (keys) ctx->kmin.reserve(sizeof(unsigned) * 3 * std::max(nshot, 1));
     if (!warm_loaded) {
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + tpb - 1) / tpb)), dim3(tpb), 0,
@@ -447,7 +593,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     hipLaunchKernelGGL(seed_sources, dim3((nshot + 63) / 64), dim3(64), 0, ctx->stream, dsrc,
                        nshot, block_model(ctx, B), ctx->T.as<float>(), ctx->flags.as<unsigned>(),
                        ctx->list[0].as<unsigned>(), cnt, ctx->dx, ctx->dy, ctx->dz, B, g.nbx,
-                       g.nby, g.vol, g.nblk);
+                       g.nby, g.vol, g.nblk, keys, ctx->kmin.as<unsigned>());
     ST_HIP(hipGetLastError());
 
     FimArgs A;
@@ -464,6 +610,10 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     A.h = ctx->dx;
     A.eps = o.eps;
     A.max_inner = o.max_inner;
+    A.keys = keys;
+    A.kmin = ctx->kmin.as<unsigned>();
+    A.nshot = nshot;
+    A.band = o.band;
 
     int grid = o.grid;
     if (grid <= 0) grid = 256 * 16;
@@ -497,19 +647,52 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     ST_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     if (stats) {
         ST_HIP(hipMemcpyAsync(hcnt + 4, cnt + 3, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
-        ST_HIP(hipMemcpyAsync(hcnt + 8, dstats, 2 * sizeof(unsigned long long),
+        ST_HIP(hipMemcpyAsync(hcnt + 8, dstats, 3 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
         const unsigned long long *hs = reinterpret_cast<const unsigned long long *>(hcnt + 8);
         stats->rounds += hcnt[4];
         stats->visits += (long long)hs[0];
         stats->inner_iters += (long long)hs[1];
+        stats->deferred += (long long)hs[2];
         stats->batches += 1;
         float ms = 0.f;
         ST_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         stats->solve_ms += ms;
-        ST_HIP(hipMemsetAsync(dstats, 0, 2 * sizeof(unsigned long long), ctx->stream));
+        ST_HIP(hipMemsetAsync(dstats, 0, 3 * sizeof(unsigned long long), ctx->stream));
     }
+}
+
+// Warm values + keys from caller volumes (device, nshot x npts, z fastest).
+void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nshot, int B,
+                    float delta, float r_cold, bool values, float *dkeys) {
+    const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+    FinePred fp{dpred, ctx->nx, ctx->nz, (long long)ctx->nx * ctx->ny * ctx->nz};
+    if (!values) {
+        const long long n4 = (long long)nshot * g.vol / 4;
+        hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ctx->T.as<float4>(), n4);
+    }
+    hipLaunchKernelGGL(warm_fill<FinePred>, dim3((unsigned)(g.nblk * nshot)), dim3(64), 0,
+                       ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
+                       g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
+                       r_cold, values ? 1 : 0, fp);
+    ST_HIP(hipGetLastError());
+}
+
+// Per-shot judge residuals (host array of nshot floats).
+void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res) {
+    const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+    ctx->judge.reserve(sizeof(unsigned) * std::max(nshot, 1));
+    ST_HIP(hipMemsetAsync(ctx->judge.ptr, 0, sizeof(unsigned) * nshot, ctx->stream));
+    const long long n = (long long)nshot * g.vol;
+    hipLaunchKernelGGL(judge_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->T.as<float>(), block_model(ctx, B), dsrc, nshot, B, g.nbx, g.nby, g.n4x,
+                       g.n4y, g.n4z, g.vol, ctx->dx, ctx->dy, ctx->dz, ctx->judge.as<unsigned>());
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipMemcpyAsync(res, ctx->judge.ptr, sizeof(float) * nshot, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 void launch_extract(st_ctx *ctx, float *dout, int nshot, int B) {

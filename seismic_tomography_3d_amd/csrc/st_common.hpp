@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -100,6 +101,7 @@ struct st_ctx {
     bool has_model = false;
     int nx = 0, ny = 0, nz = 0;
     float dx = 0, dy = 0, dz = 0;
+    float smin = 0, smax = 0;          // slowness range of the model
     st::DevBuf slow_zxy;               // slowness, z fastest (for updates)
     st::DevBuf slow_blk[2];            // block layouts for B = 4, 8
     bool blk_valid[2] = {false, false};
@@ -110,6 +112,7 @@ struct st_ctx {
     st::DevBuf list[2];                // active lists (by round parity)
     st::DevBuf counters;               // list counts, stats
     st::DevBuf src, rcv, picks, scratch;
+    st::DevBuf keys, kmin, judge, pred;   // warm-start state
     void *host_pinned = nullptr;       // small pinned staging area
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 

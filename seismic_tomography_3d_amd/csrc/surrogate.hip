@@ -2,7 +2,9 @@
 #include "st_common.hpp"
 
 namespace st {
-bool surrogate_warm_fill(st_ctx *, const float *, int, int, float) { return false; }
-int surrogate_judge(st_ctx *, const float *, int, int, float, int *) { return 0; }
+bool surrogate_warm_fill(st_ctx *, const float *, int, int, float, float, bool, float *) {
+    return false;
+}
+float surrogate_cold_radius(st_ctx *ctx) { return 2.0f * 8 * ctx->dx; }
 void surrogate_release(st_ctx *) {}
 }  // namespace st
