@@ -49,9 +49,11 @@ typedef struct {
                            scheduled block (0 = default, <0 = off)             */
     float cold_radius;  /* warm starts: cells closer than this to the source
                            start cold (m, 0 = default)                         */
-    int warm_values;    /* warm starts: 1 initialise T = (1+delta)*prediction,
-                           0 keep T cold and use the prediction only to order
-                           block visits                                        */
+    int warm_values;    /* warm starts: 1 initialise T = (1+delta)*prediction
+                           (an upper bound only if the prediction errs by less
+                           than delta; the judge re-solves the others cold),
+                           0 (default) keep T cold and use the prediction only
+                           to order block visits (exact by construction)       */
 } st_fim_opts;
 
 typedef struct {
@@ -166,6 +168,34 @@ int st_cgls_csr(st_ctx *ctx, const long long *rowptr, const int *cols,
  * dm and velocity (nullable, nx*ny*nz) receive copies. */
 int st_model_update(st_ctx *ctx, const float *xdm, int smooth, int samples,
                     float stdv, float *dm, float *velocity);
+
+/* ---- learned warm start (no counterpart in the reference) ------------- */
+/* A 3D U-Net on an nc^3 node cube (nc a power of two, 32 by default)
+ * predicting y = T*3000/max(D, dmin) from the slowness, the source
+ * position and the straight-ray mean slowness; st_fim_solve with opts.warm = 1 uses it to order block visits
+ * and to initialise (1+delta)*T.  Training and inference run on the device
+ * with BF16 MFMA GEMMs (FP32 accumulation). */
+int st_surrogate_init(st_ctx *ctx, int nc, unsigned seed);
+int st_surrogate_nparams(st_ctx *ctx, long long *n);
+/* copy parameters out (get) and/or in (set); either may be NULL */
+int st_surrogate_params(st_ctx *ctx, float *get, const float *set);
+/* forward on X[ns, nc^3, 6] -> Y[ns, nc^3]; with target and mask also the
+ * weighted MSE loss and its parameter gradient (grad: nparams) */
+int st_surrogate_eval(st_ctx *ctx, const float *X, int ns, const float *target,
+                      const float *mask, float *Y, double *loss, float *grad);
+/* training set on the current model grid, solved by the plain FIM */
+int st_surrogate_make_data(st_ctx *ctx, int n_models, int shots_per_model,
+                           float zmax_src, unsigned seed, float grad_fraction);
+/* Adam; history (nullable): per epoch [train loss, validation loss] */
+int st_surrogate_train(st_ctx *ctx, int epochs, int batch, float lr, int n_val,
+                       unsigned seed, float *history);
+int st_surrogate_save(st_ctx *ctx, const char *path);
+int st_surrogate_load(st_ctx *ctx, const char *path);
+/* network output y on the node cube for the given sources (diagnostics) */
+int st_surrogate_predict(st_ctx *ctx, const float *src_xyz, int nshot, float *y);
+/* diagnostic: C = A B with the BF16 MFMA GEMM of the network */
+int st_nn_gemm(st_ctx *ctx, const float *A, const float *B, float *C, int M,
+               int N, int K);
 
 #ifdef __cplusplus
 }

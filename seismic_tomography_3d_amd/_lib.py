@@ -81,6 +81,21 @@ SIGNATURES = [
                                    ctypes.c_int, c_float_p, ctypes.c_int, c_float_p]),
     ("st_model_update", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_float, c_float_p, c_float_p]),
+    ("st_surrogate_init", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint]),
+    ("st_surrogate_nparams", ctypes.c_int, [ctypes.c_void_p, c_ll_p]),
+    ("st_surrogate_params", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p]),
+    ("st_surrogate_eval", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                         c_float_p, c_float_p, ctypes.POINTER(ctypes.c_double),
+                                         c_float_p]),
+    ("st_surrogate_make_data", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_float, ctypes.c_uint, ctypes.c_float]),
+    ("st_surrogate_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_float, ctypes.c_int, ctypes.c_uint, c_float_p]),
+    ("st_surrogate_save", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("st_surrogate_load", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("st_surrogate_predict", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p]),
+    ("st_nn_gemm", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int]),
 ]
 
 _lib = None

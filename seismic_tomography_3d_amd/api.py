@@ -205,3 +205,69 @@ class Context:
         self._check(self.lib.st_model_update(self.h, _ptr(x), int(bool(smooth)), samples, stdv,
                                              _ptr(dm), _ptr(vel)))
         return dm, vel
+
+    # ---- learned warm start ----
+    def surrogate_init(self, nc=32, seed=0):
+        self._check(self.lib.st_surrogate_init(self.h, nc, seed))
+        n = ctypes.c_longlong()
+        self._check(self.lib.st_surrogate_nparams(self.h, ctypes.byref(n)))
+        self.nc = nc
+        return n.value
+
+    def surrogate_params(self, set_to=None):
+        n = ctypes.c_longlong()
+        self._check(self.lib.st_surrogate_nparams(self.h, ctypes.byref(n)))
+        out = np.empty(n.value, np.float32)
+        s = _f32(set_to) if set_to is not None else None
+        self._check(self.lib.st_surrogate_params(self.h, _ptr(out), _ptr(s)))
+        return out
+
+    def surrogate_eval(self, X, target=None, mask=None):
+        """X[ns, nc^3, 6] -> (Y[ns, nc^3], loss, grad)."""
+        X = _f32(X)
+        ns = X.shape[0]
+        V = X.size // (ns * 6)
+        Y = np.empty((ns, V), np.float32)
+        loss = ctypes.c_double()
+        grad = None
+        t = m = None
+        if target is not None:
+            t, m = _f32(target), _f32(mask)
+            n = ctypes.c_longlong()
+            self._check(self.lib.st_surrogate_nparams(self.h, ctypes.byref(n)))
+            grad = np.empty(n.value, np.float32)
+        self._check(self.lib.st_surrogate_eval(self.h, _ptr(X), ns, _ptr(t), _ptr(m), _ptr(Y),
+                                               ctypes.byref(loss), _ptr(grad)))
+        return Y, (loss.value if target is not None else None), grad
+
+    def surrogate_make_data(self, n_models, shots_per_model=4, zmax_src=2000.0, seed=5,
+                            grad_fraction=0.5):
+        self._check(self.lib.st_surrogate_make_data(self.h, n_models, shots_per_model, zmax_src,
+                                                    seed, grad_fraction))
+
+    def surrogate_train(self, epochs, batch=8, lr=1e-3, n_val=64, seed=0):
+        hist = np.zeros((epochs, 2), np.float32)
+        self._check(self.lib.st_surrogate_train(self.h, epochs, batch, lr, n_val, seed, _ptr(hist)))
+        return hist
+
+    def surrogate_save(self, path):
+        self._check(self.lib.st_surrogate_save(self.h, str(path).encode()))
+
+    def surrogate_load(self, path):
+        self._check(self.lib.st_surrogate_load(self.h, str(path).encode()))
+
+    def surrogate_predict(self, src):
+        src = _f32(src, (-1, 3))
+        nc = getattr(self, "nc", 32)
+        y = np.empty((src.shape[0], nc ** 3), np.float32)
+        self._check(self.lib.st_surrogate_predict(self.h, _ptr(src), src.shape[0], _ptr(y)))
+        return y
+
+    def nn_gemm(self, A, B):
+        A = _f32(A)
+        B = _f32(B)
+        M, K = A.shape
+        N = B.shape[1]
+        C = np.empty((M, N), np.float32)
+        self._check(self.lib.st_nn_gemm(self.h, _ptr(A), _ptr(B), _ptr(C), M, N, K))
+        return C

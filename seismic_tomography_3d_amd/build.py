@@ -36,7 +36,7 @@ HIP_FLAGS = [
 # files whose results must be bit-identical to the CPU formulas
 NO_CONTRACT = {"picks.hip", "tomo.hip"}
 
-HIP_SOURCES = ["api.hip", "fim.hip", "picks.hip", "tomo.hip", "surrogate.hip"]
+HIP_SOURCES = ["api.hip", "fim.hip", "picks.hip", "tomo.hip", "surrogate.hip", "nn.hip"]
 HOST_SOURCES = ["file_manager.cpp", "geometry.cpp", "block_fim.cpp", "least_squares.cpp"]
 
 

@@ -94,7 +94,7 @@ void st_fim_default_opts(st_fim_opts *o) {
     o->use_graph = 0;
     o->band = 0.0f;
     o->cold_radius = 0.0f;
-    o->warm_values = 1;
+    o->warm_values = 0;
 }
 
 int st_ctx_create(int device, st_ctx **out) {

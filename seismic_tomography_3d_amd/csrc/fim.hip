@@ -474,6 +474,26 @@ struct FinePred {  // caller volumes, z fastest, one per shot
     }
 };
 
+struct CoarsePred {  // network output y on an nc^3 node cube spanning the model
+    const float *Q;
+    int nc;
+    float ux, uy, uz;    // (nc - 1) / extent per axis
+    float inv_vref, dmin;
+    __device__ float operator()(int s, int, int, int, float px, float py, float pz, float dist) const {
+        const float u = px * ux, v = py * uy, w = pz * uz;
+        const int i0 = min((int)u, nc - 2), j0 = min((int)v, nc - 2), k0 = min((int)w, nc - 2);
+        const float fu = u - i0, fv = v - j0, fw = w - k0;
+        const float *q = Q + (long long)s * nc * nc * nc;
+        auto at = [&](int i, int j, int k) { return q[i + nc * (j + nc * k)]; };
+        const float c00 = at(i0, j0, k0) * (1 - fu) + at(i0 + 1, j0, k0) * fu;
+        const float c10 = at(i0, j0 + 1, k0) * (1 - fu) + at(i0 + 1, j0 + 1, k0) * fu;
+        const float c01 = at(i0, j0, k0 + 1) * (1 - fu) + at(i0 + 1, j0, k0 + 1) * fu;
+        const float c11 = at(i0, j0 + 1, k0 + 1) * (1 - fu) + at(i0 + 1, j0 + 1, k0 + 1) * fu;
+        const float c0 = c00 * (1 - fv) + c10 * fv, c1 = c01 * (1 - fv) + c11 * fv;
+        return (c0 * (1 - fw) + c1 * fw) * fmaxf(dist, dmin) * inv_vref;
+    }
+};
+
 __device__ __forceinline__ float wave_min(float v) {
     for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
     return v;
@@ -677,6 +697,26 @@ void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nsho
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
                        g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
                        r_cold, values ? 1 : 0, fp);
+    ST_HIP(hipGetLastError());
+}
+
+// Warm values + keys from the network's node-cube output (device,
+// nshot x nc^3): T = y * max(D, dmin) / vref.
+void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dmin,
+                      const float *dsrc, int nshot, int B, float delta, float r_cold, bool values,
+                      float *dkeys) {
+    const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+    CoarsePred cp{dQ, nc, (nc - 1) / ((ctx->nx - 1) * ctx->dx), (nc - 1) / ((ctx->ny - 1) * ctx->dy),
+                  (nc - 1) / ((ctx->nz - 1) * ctx->dz), 1.0f / vref, dmin};
+    if (!values) {
+        const long long n4 = (long long)nshot * g.vol / 4;
+        hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ctx->T.as<float4>(), n4);
+    }
+    hipLaunchKernelGGL(warm_fill<CoarsePred>, dim3((unsigned)(g.nblk * nshot)), dim3(64), 0,
+                       ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
+                       g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
+                       r_cold, values ? 1 : 0, cp);
     ST_HIP(hipGetLastError());
 }
 

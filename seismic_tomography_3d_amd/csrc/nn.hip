@@ -1,0 +1,368 @@
+// nn.hip -- hand-written training and inference kernels of the warm-start
+// network: BF16 MFMA GEMM with FP32 accumulation, 3D im2col / col2im,
+// average pooling, nearest upsampling, channel concat, bias/ReLU, MSE loss
+// and Adam.  Activations are channels-last [sample][z][y][x][c], voxel
+// v = x + n*(y + n*z) on an n^3 cube.
+#include "nn.hpp"
+
+namespace st {
+namespace nn {
+
+// ---------------------------------------------------------------------------
+// GEMM  C[M,N] = A[M,K] * B[K,N]  (BF16 operands, FP32 accumulate)
+//   ta = 0: A(m,k) = A[m*lda + k]     ta = 1: A(m,k) = A[k*lda + m]
+//   tb = 0: B(k,n) = B[k*ldb + n]     tb = 1: B(k,n) = B[n*ldb + k]
+//   mode 0: C = act(acc + bias) (and a BF16 copy when C16 is set)
+//   mode 1: C += acc with atomics (split-K over blockIdx.z)
+// 256 threads = 4 waves; 64x64 tile per workgroup, 32x32 per wave as 2x2
+// v_mfma_f32_16x16x16_bf16 tiles; K staged through LDS in steps of 32.
+// ---------------------------------------------------------------------------
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
+    __shared__ unsigned short As[64][40];  // [m][k]
+    __shared__ unsigned short Bs[64][40];  // [n][k]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long long m0 = (long long)blockIdx.y * 64;
+    const int n0 = blockIdx.x * 64;
+    const int kchunk = (((g.K + g.ksplit - 1) / g.ksplit) + 31) / 32 * 32;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(g.K, kbeg + kchunk);
+    f4v acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
+        for (int e = tid; e < 2048; e += 256) {
+            int mm, kk;
+            if (!g.ta) { mm = e >> 5; kk = e & 31; } else { kk = e >> 6; mm = e & 63; }
+            const long long m = m0 + mm;
+            const int k = k0 + kk;
+            unsigned short v = 0;
+            if (m < g.M && k < kend)
+                v = g.ta ? g.A[(long long)k * g.lda + m] : g.A[m * g.lda + k];
+            As[mm][kk] = v;
+        }
+        for (int e = tid; e < 2048; e += 256) {
+            int nn, kk;
+            if (!g.tb) { kk = e >> 6; nn = e & 63; } else { nn = e >> 5; kk = e & 31; }
+            const int n = n0 + nn, k = k0 + kk;
+            unsigned short v = 0;
+            if (n < g.N && k < kend)
+                v = g.tb ? g.B[(long long)n * g.ldb + k] : g.B[(long long)k * g.ldb + n];
+            Bs[nn][kk] = v;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < 32; ks += 16) {
+            const int kq = ks + 4 * (lane >> 4);
+            s4v a[2], b[2];
+            for (int i = 0; i < 2; i++) {
+                const int r = wm + i * 16 + (lane & 15);
+                a[i] = s4v{(short)As[r][kq], (short)As[r][kq + 1], (short)As[r][kq + 2],
+                            (short)As[r][kq + 3]};
+            }
+            for (int j = 0; j < 2; j++) {
+                const int c = wn + j * 16 + (lane & 15);
+                b[j] = s4v{(short)Bs[c][kq], (short)Bs[c][kq + 1], (short)Bs[c][kq + 2],
+                           (short)Bs[c][kq + 3]};
+            }
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            for (int r = 0; r < 4; r++) {
+                const long long m = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[i][j][r];
+                if (g.mode == 1) {
+                    atomicAdd(&g.C[m * g.ldc + n], v);
+                } else {
+                    if (g.bias) v += g.bias[n];
+                    if (g.relu) v = fmaxf(v, 0.f);
+                    if (g.C) g.C[m * g.ldc + n] = v;
+                    if (g.C16) g.C16[m * g.ldc + n] = f2bf(v);
+                }
+            }
+}
+
+void gemm(hipStream_t st, const GemmArgs &g) {
+    dim3 grid((g.N + 63) / 64, (unsigned)((g.M + 63) / 64), g.ksplit);
+    hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, st, g);
+    ST_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// elementwise / layout kernels
+// ---------------------------------------------------------------------------
+#define GRID_STRIDE(i, n) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+         i += (long long)gridDim.x * blockDim.x)
+
+// col[(s,v), tap*C + c] = X[s, v + off(tap), c] (zero outside), BF16
+__global__ void im2col3(const float *__restrict__ X, unsigned short *__restrict__ col, int ns,
+                        int n, int C, int ksz) {
+    const long long V = (long long)n * n * n;
+    const int taps = ksz * ksz * ksz, h = ksz / 2;
+    const long long total = (long long)ns * V * taps * C;
+    GRID_STRIDE(i, total) {
+        const int c = (int)(i % C);
+        long long r = i / C;
+        const int tap = (int)(r % taps);
+        const long long sv = r / taps;
+        const long long s = sv / V;
+        const int v = (int)(sv - s * V);
+        const int x = v % n, y = (v / n) % n, z = v / (n * n);
+        const int tx = tap % ksz - h, ty = (tap / ksz) % ksz - h, tz = tap / (ksz * ksz) - h;
+        const int xx = x + tx, yy = y + ty, zz = z + tz;
+        float val = 0.f;
+        if (xx >= 0 && yy >= 0 && zz >= 0 && xx < n && yy < n && zz < n)
+            val = X[(s * V + xx + (long long)n * (yy + (long long)n * zz)) * C + c];
+        col[i] = f2bf(val);
+    }
+}
+
+// dX[s, v, c] = sum_tap dcol[(s, v - off(tap)), tap*C + c]
+__global__ void col2im3(const float *__restrict__ dcol, float *__restrict__ dX, int ns, int n,
+                        int C, int ksz) {
+    const long long V = (long long)n * n * n;
+    const int taps = ksz * ksz * ksz, h = ksz / 2;
+    const long long total = (long long)ns * V * C;
+    GRID_STRIDE(i, total) {
+        const int c = (int)(i % C);
+        const long long sv = i / C;
+        const long long s = sv / V;
+        const int v = (int)(sv - s * V);
+        const int x = v % n, y = (v / n) % n, z = v / (n * n);
+        float acc = 0.f;
+        for (int tap = 0; tap < taps; tap++) {
+            const int tx = tap % ksz - h, ty = (tap / ksz) % ksz - h, tz = tap / (ksz * ksz) - h;
+            const int xx = x - tx, yy = y - ty, zz = z - tz;
+            if (xx < 0 || yy < 0 || zz < 0 || xx >= n || yy >= n || zz >= n) continue;
+            const long long src = s * V + xx + (long long)n * (yy + (long long)n * zz);
+            acc += dcol[(src * taps + tap) * C + c];
+        }
+        dX[i] = acc;
+    }
+}
+
+// 2x average pooling: out n/2 cube
+__global__ void avgpool2(const float *__restrict__ X, float *__restrict__ Y, int ns, int n, int C) {
+    const int m = n / 2;
+    const long long V2 = (long long)m * m * m, V = (long long)n * n * n;
+    GRID_STRIDE(i, (long long)ns * V2 * C) {
+        const int c = (int)(i % C);
+        const long long sv = i / C;
+        const long long s = sv / V2;
+        const int v = (int)(sv - s * V2);
+        const int x = v % m, y = (v / m) % m, z = v / (m * m);
+        float acc = 0.f;
+        for (int d = 0; d < 8; d++) {
+            const int xx = 2 * x + (d & 1), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d >> 2);
+            acc += X[(s * V + xx + (long long)n * (yy + (long long)n * zz)) * C + c];
+        }
+        Y[i] = 0.125f * acc;
+    }
+}
+
+__global__ void avgpool2_bwd(const float *__restrict__ dY, float *__restrict__ dX, int ns, int n,
+                             int C) {
+    const int m = n / 2;
+    const long long V2 = (long long)m * m * m, V = (long long)n * n * n;
+    GRID_STRIDE(i, (long long)ns * V * C) {
+        const int c = (int)(i % C);
+        const long long sv = i / C;
+        const long long s = sv / V;
+        const int v = (int)(sv - s * V);
+        const int x = v % n, y = (v / n) % n, z = v / (n * n);
+        dX[i] = 0.125f * dY[(s * V2 + x / 2 + (long long)m * (y / 2 + (long long)m * (z / 2))) * C + c];
+    }
+}
+
+// nearest 2x upsampling of X (m cube) into channels [off, off+C) of Y (2m
+// cube, Ctot channels)
+__global__ void upsample2(const float *__restrict__ X, float *__restrict__ Y, int ns, int m, int C,
+                          int Ctot, int off) {
+    const int n = 2 * m;
+    const long long V = (long long)n * n * n, V2 = (long long)m * m * m;
+    GRID_STRIDE(i, (long long)ns * V * C) {
+        const int c = (int)(i % C);
+        const long long sv = i / C;
+        const long long s = sv / V;
+        const int v = (int)(sv - s * V);
+        const int x = v % n, y = (v / n) % n, z = v / (n * n);
+        Y[sv * Ctot + off + c] = X[(s * V2 + x / 2 + (long long)m * (y / 2 + (long long)m * (z / 2))) * C + c];
+    }
+}
+
+__global__ void upsample2_bwd(const float *__restrict__ dY, float *__restrict__ dX, int ns, int m,
+                              int C, int Ctot, int off) {
+    const int n = 2 * m;
+    const long long V = (long long)n * n * n, V2 = (long long)m * m * m;
+    GRID_STRIDE(i, (long long)ns * V2 * C) {
+        const int c = (int)(i % C);
+        const long long sv = i / C;
+        const long long s = sv / V2;
+        const int v = (int)(sv - s * V2);
+        const int x = v % m, y = (v / m) % m, z = v / (m * m);
+        float acc = 0.f;
+        for (int d = 0; d < 8; d++) {
+            const int xx = 2 * x + (d & 1), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d >> 2);
+            acc += dY[(s * V + xx + (long long)n * (yy + (long long)n * zz)) * Ctot + off + c];
+        }
+        dX[i] = acc;
+    }
+}
+
+// copy X (C channels) into channels [off, off+C) of Y (Ctot), or back
+__global__ void chan_copy(const float *__restrict__ X, float *__restrict__ Y, long long rows, int C,
+                          int Ctot, int off, int backward) {
+    GRID_STRIDE(i, rows * C) {
+        const long long r = i / C;
+        const int c = (int)(i - r * C);
+        if (!backward) Y[r * Ctot + off + c] = X[i];
+        else Y[i] = X[r * Ctot + off + c];  // here X is the wide gradient
+    }
+}
+
+// dY *= (Y > 0), and a BF16 copy
+__global__ void relu_bwd(float *__restrict__ dY, const float *__restrict__ Y,
+                         unsigned short *__restrict__ d16, long long n, int relu) {
+    GRID_STRIDE(i, n) {
+        float d = dY[i];
+        if (relu && !(Y[i] > 0.f)) d = 0.f;
+        dY[i] = d;
+        d16[i] = f2bf(d);
+    }
+}
+
+// db[c] += sum_r dY[r, c]
+__global__ void bias_grad(const float *__restrict__ dY, float *__restrict__ db, long long rows, int C) {
+    __shared__ float sh[256];
+    const int c = blockIdx.y;
+    float acc = 0.f;
+    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < rows; r += (long long)gridDim.x * 256)
+        acc += dY[r * C + c];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&db[c], sh[0]);
+}
+
+// weighted MSE: loss = sum m (p - t)^2 / sum m; grad = 2 m (p - t) / sum m
+__global__ void mse(const float *__restrict__ p, const float *__restrict__ t,
+                    const float *__restrict__ m, float *__restrict__ grad, double *__restrict__ acc,
+                    long long n) {
+    __shared__ double sl[256], sw[256];
+    double l = 0.0, ws = 0.0;
+    GRID_STRIDE(i, n) {
+        const float d = p[i] - t[i];
+        l += (double)m[i] * d * d;
+        ws += m[i];
+        grad[i] = 2.f * m[i] * d;  // scaled by 1/sum m afterwards
+    }
+    sl[threadIdx.x] = l;
+    sw[threadIdx.x] = ws;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sl[threadIdx.x] += sl[threadIdx.x + o]; sw[threadIdx.x] += sw[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { atomicAdd(&acc[0], sl[0]); atomicAdd(&acc[1], sw[0]); }
+}
+
+__global__ void scale_by_inv(float *__restrict__ g, const double *__restrict__ acc, long long n) {
+    const float s = acc[1] > 0 ? (float)(1.0 / acc[1]) : 0.f;
+    GRID_STRIDE(i, n) g[i] *= s;
+}
+
+__global__ void adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                     float *__restrict__ v, unsigned short *__restrict__ w16, long long n, float lr,
+                     float b1, float b2, float c1, float c2, float eps) {
+    GRID_STRIDE(i, n) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (w16) w16[i] = f2bf(wi);
+    }
+}
+
+__global__ void to_bf16(const float *__restrict__ x, unsigned short *__restrict__ y, long long n) {
+    GRID_STRIDE(i, n) y[i] = f2bf(x[i]);
+}
+
+static unsigned blocks_for(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b < 65536 ? b : 65536);
+}
+
+// ---------------------------------------------------------------------------
+// host wrappers
+// ---------------------------------------------------------------------------
+void im2col(hipStream_t st, const float *X, unsigned short *col, int ns, int n, int C, int ksz) {
+    const long long tot = (long long)ns * n * n * n * ksz * ksz * ksz * C;
+    hipLaunchKernelGGL(im2col3, dim3(blocks_for(tot)), dim3(256), 0, st, X, col, ns, n, C, ksz);
+}
+void col2im(hipStream_t st, const float *dcol, float *dX, int ns, int n, int C, int ksz) {
+    const long long tot = (long long)ns * n * n * n * C;
+    hipLaunchKernelGGL(col2im3, dim3(blocks_for(tot)), dim3(256), 0, st, dcol, dX, ns, n, C, ksz);
+}
+void pool(hipStream_t st, const float *X, float *Y, int ns, int n, int C) {
+    hipLaunchKernelGGL(avgpool2, dim3(blocks_for((long long)ns * n * n * n / 8 * C)), dim3(256), 0,
+                       st, X, Y, ns, n, C);
+}
+void pool_bwd(hipStream_t st, const float *dY, float *dX, int ns, int n, int C) {
+    hipLaunchKernelGGL(avgpool2_bwd, dim3(blocks_for((long long)ns * n * n * n * C)), dim3(256), 0,
+                       st, dY, dX, ns, n, C);
+}
+void upsample(hipStream_t st, const float *X, float *Y, int ns, int m, int C, int Ctot, int off) {
+    hipLaunchKernelGGL(upsample2, dim3(blocks_for((long long)ns * 8 * m * m * m * C)), dim3(256), 0,
+                       st, X, Y, ns, m, C, Ctot, off);
+}
+void upsample_bwd(hipStream_t st, const float *dY, float *dX, int ns, int m, int C, int Ctot, int off) {
+    hipLaunchKernelGGL(upsample2_bwd, dim3(blocks_for((long long)ns * m * m * m * C)), dim3(256), 0,
+                       st, dY, dX, ns, m, C, Ctot, off);
+}
+void chancopy(hipStream_t st, const float *X, float *Y, long long rows, int C, int Ctot, int off,
+              bool backward) {
+    hipLaunchKernelGGL(chan_copy, dim3(blocks_for(rows * C)), dim3(256), 0, st, X, Y, rows, C, Ctot,
+                       off, backward ? 1 : 0);
+}
+void relu_backward(hipStream_t st, float *dY, const float *Y, unsigned short *d16, long long n,
+                   bool relu) {
+    hipLaunchKernelGGL(relu_bwd, dim3(blocks_for(n)), dim3(256), 0, st, dY, Y, d16, n, relu ? 1 : 0);
+}
+void bias_gradient(hipStream_t st, const float *dY, float *db, long long rows, int C) {
+    hipLaunchKernelGGL(bias_grad, dim3(64, C), dim3(256), 0, st, dY, db, rows, C);
+}
+void mse_loss(hipStream_t st, const float *p, const float *t, const float *m, float *grad,
+              double *acc, long long n) {
+    hipLaunchKernelGGL(mse, dim3(blocks_for(n) < 1024 ? blocks_for(n) : 1024), dim3(256), 0, st, p,
+                       t, m, grad, acc, n);
+    hipLaunchKernelGGL(scale_by_inv, dim3(blocks_for(n)), dim3(256), 0, st, grad, acc, n);
+}
+void adam_step(hipStream_t st, float *w, const float *g, float *m, float *v, unsigned short *w16,
+               long long n, float lr, int step) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam, dim3(blocks_for(n)), dim3(256), 0, st, w, g, m, v, w16, n, lr, b1, b2,
+                       c1, c2, 1e-8f);
+}
+void bf16_copy(hipStream_t st, const float *x, unsigned short *y, long long n) {
+    hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, st, x, y, n);
+}
+
+}  // namespace nn
+}  // namespace st

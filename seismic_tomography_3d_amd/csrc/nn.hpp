@@ -1,0 +1,51 @@
+// nn.hpp -- kernels of the warm-start network (nn.hip).
+#pragma once
+
+#include "st_common.hpp"
+
+namespace st {
+namespace nn {
+
+__host__ __device__ inline unsigned short f2bf(float f) {  // round to nearest even
+#ifdef __HIP_DEVICE_COMPILE__
+    unsigned u = __float_as_uint(f);
+#else
+    unsigned u;
+    __builtin_memcpy(&u, &f, 4);
+#endif
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+struct GemmArgs {
+    const unsigned short *A;
+    const unsigned short *B;
+    float *C;
+    unsigned short *C16;
+    const float *bias;
+    long long M;
+    int N, K;
+    long long lda, ldb, ldc;
+    int ta, tb, relu, mode, ksplit;
+};
+
+void gemm(hipStream_t st, const GemmArgs &g);
+void im2col(hipStream_t st, const float *X, unsigned short *col, int ns, int n, int C, int ksz);
+void col2im(hipStream_t st, const float *dcol, float *dX, int ns, int n, int C, int ksz);
+void pool(hipStream_t st, const float *X, float *Y, int ns, int n, int C);
+void pool_bwd(hipStream_t st, const float *dY, float *dX, int ns, int n, int C);
+void upsample(hipStream_t st, const float *X, float *Y, int ns, int m, int C, int Ctot, int off);
+void upsample_bwd(hipStream_t st, const float *dY, float *dX, int ns, int m, int C, int Ctot, int off);
+void chancopy(hipStream_t st, const float *X, float *Y, long long rows, int C, int Ctot, int off,
+              bool backward);
+void relu_backward(hipStream_t st, float *dY, const float *Y, unsigned short *d16, long long n,
+                   bool relu);
+void bias_gradient(hipStream_t st, const float *dY, float *db, long long rows, int C);
+void mse_loss(hipStream_t st, const float *p, const float *t, const float *m, float *grad,
+              double *acc, long long n);
+void adam_step(hipStream_t st, float *w, const float *g, float *m, float *v, unsigned short *w16,
+               long long n, float lr, int step);
+void bf16_copy(hipStream_t st, const float *x, unsigned short *y, long long n);
+
+}  // namespace nn
+}  // namespace st
