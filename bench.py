@@ -130,7 +130,8 @@ def main():
 
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(shots shard across GPUs by design, see DESIGN.md)"}))
+                                   "(shots shard across GPUs by design: one shot "
+                                   "range per rank, no collective in the solve)"}))
         sys.exit(2)
 
     from seismic_tomography_3d_amd import Context, synth, tune
@@ -151,10 +152,13 @@ def main():
         if a.warm == "on" and not have:
             raise SystemExit("--warm on but no trained network is available")
         warm = 1 if have else 0
-    opts = dict(block=params["block"], grid=params["grid"], max_inner=params["max_inner"],
-                batch=a.batch, warm=warm)
+    cold_opts = dict(block=params["block"], grid=params["grid"],
+                     max_inner=params["max_inner"], batch=a.batch)
+    opts = dict(cold_opts)
+    if warm:
+        wp = tune.best_warm_params(ctx, src, rcv, cold_opts, log=log)
+        opts.update(wp, warm=1)
     log(f"options {opts}")
-
     for _ in range(a.warmup):
         ctx.fim_solve(src, rcv, **opts)
     times, stats = [], None
@@ -166,16 +170,32 @@ def main():
     ms = 1e3 * float(np.median(times))
     value = len(src) / (ms / 1e3)
 
-    # plain FIM: same kernels, cold start, one shot at a time (bounded sample)
+    # the same batched solver without the network (what the network gains)
+    batched_cold = None
+    if warm:
+        ctx.fim_solve(src, rcv, **cold_opts)
+        t0 = time.perf_counter()
+        _, _, cst = ctx.fim_solve(src, rcv, **cold_opts)
+        cdt = time.perf_counter() - t0
+        batched_cold = {"value": len(src) / cdt, "unit": "shot-solves/s", "ms_per_step": 1e3 * cdt,
+                        "visits": cst["visits"], "inner_iters": cst["inner_iters"],
+                        "rounds": cst["rounds"]}
+        log(f"batched cold: {batched_cold}")
+
+    # plain FIM: same kernels, cold start, one shot at a time, its own tuned
+    # launch shape (bounded sample)
     npl = min(a.plain_shots, len(src))
-    popts = dict(opts, batch=1, warm=0)
+    pparams = tune.best_params(ctx, src[:npl], rcv, block=a.block, batch=1, log=log,
+                               candidates=[dict(block=B, grid=g, max_inner=16, batch=1)
+                                           for B in (4, 8) for g in (256, 512, 1024, 2048)])
+    popts = dict(pparams, batch=1, warm=0)
     ctx.fim_solve(src[:2], rcv, **popts)
     t0 = time.perf_counter()
     _, _, pst = ctx.fim_solve(src[:npl], rcv, **popts)
     pdt = time.perf_counter() - t0
     plain = {"value": npl / pdt, "unit": "shot-solves/s", "shots": npl,
              "ms_per_shot": 1e3 * pdt / npl, "rounds_per_shot": pst["rounds"] / npl,
-             "visits_per_shot": pst["visits"] / npl}
+             "visits_per_shot": pst["visits"] / npl, "options": popts}
     log(f"plain: {plain}")
 
     # roofline of the FIM round kernel over the last timed step
@@ -201,9 +221,12 @@ def main():
                                "1500-4000 m/s + -10% anomaly, 400 surface receivers",
                    "shots": len(src), "receivers": len(rcv), "block": B,
                    "batch": a.batch, "grid": params["grid"],
-                   "max_inner": params["max_inner"], "warm_start": bool(warm)},
+                   "max_inner": params["max_inner"], "warm_start": bool(warm),
+                   "options": opts},
         "plain_fim": plain,
         "speedup_vs_plain": value / plain["value"],
+        "batched_cold_fim": batched_cold,
+        "network_gain": (value / batched_cold["value"]) if batched_cold else None,
         "fim_stats_per_step": stats,
         "roofline": roof,
     }

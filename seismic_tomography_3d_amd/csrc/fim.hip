@@ -644,13 +644,51 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     ST_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     long long r = 0;
     int slot = 0, pending = -1;
+    // Graph replay: six rounds (one period of the list parity and counter
+    // rotation) captured once per launch configuration and replayed; the
+    // round kernels clear their own counters, so the graph holds kernel nodes
+    // only.
+    hipGraphExec_t exec = nullptr;
+    if (o.use_graph) {
+        long long key = 1469598103934665603LL;
+        auto mix = [&](long long v) { key = (key ^ v) * 1099511628211LL; };
+        mix((long long)(uintptr_t)A.T); mix((long long)(uintptr_t)A.S); mix((long long)(uintptr_t)A.keys);
+        mix((long long)(uintptr_t)A.flags); mix((long long)(uintptr_t)A.list0);
+        mix((long long)(uintptr_t)A.list1); mix((long long)(uintptr_t)A.kmin);
+        mix(nshot); mix(B); mix(grid); mix(A.nblk); mix(A.max_inner);
+        mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
+        mix((long long)__builtin_bit_cast(unsigned, A.h));
+        if (!ctx->graph_exec || ctx->graph_key != key) {
+            if (ctx->graph_exec) ST_HIP(hipGraphExecDestroy(ctx->graph_exec));
+            if (ctx->graph) ST_HIP(hipGraphDestroy(ctx->graph));
+            ctx->graph_exec = nullptr;
+            ctx->graph = nullptr;
+            ST_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < 6; k++) {
+                A.round = k;
+                if (B == 8)
+                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64), 0, ctx->stream, A);
+                else
+                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64), 0, ctx->stream, A);
+            }
+            ST_HIP(hipStreamEndCapture(ctx->stream, &ctx->graph));
+            ST_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+            ctx->graph_key = key;
+        }
+        exec = ctx->graph_exec;
+    }
+    const int Kg = exec ? 6 * std::max(1, K / 6) : K;
     for (;;) {
-        for (int k = 0; k < K; k++, r++) {
-            A.round = (int)(r % 6);
-            if (B == 8)
-                hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64), 0, ctx->stream, A);
-            else
-                hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64), 0, ctx->stream, A);
+        if (exec) {
+            for (int k = 0; k < Kg; k += 6, r += 6) ST_HIP(hipGraphLaunch(exec, ctx->stream));
+        } else {
+            for (int k = 0; k < Kg; k++, r++) {
+                A.round = (int)(r % 6);
+                if (B == 8)
+                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64), 0, ctx->stream, A);
+                else
+                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64), 0, ctx->stream, A);
+            }
         }
         ST_HIP(hipGetLastError());
         ST_HIP(hipMemcpyAsync(&hcnt[slot], &cnt[r % 3], sizeof(unsigned), hipMemcpyDeviceToHost,

@@ -36,15 +36,20 @@ def attach(ctx, case=None, path=None, log=None):
 
 def train(ctx, n_models=2000, spm=4, epochs=20, batch=8, lr=2e-3, n_val=256, seed=5,
           nc=32, log=print):
+    """Generates the training set on the device and runs Adam epoch by epoch
+    with a cosine learning-rate decay (lr -> lr/20)."""
     ctx.surrogate_init(nc, seed=seed)
     t0 = time.time()
     ctx.surrogate_make_data(n_models, shots_per_model=spm, zmax_src=2000.0, seed=seed)
     t1 = time.time()
     log(f"training data: {n_models * spm} samples in {t1 - t0:.1f}s")
-    hist = ctx.surrogate_train(epochs, batch=batch, lr=lr, n_val=n_val, seed=seed + 1)
-    log(f"trained {epochs} epochs in {time.time() - t1:.1f}s")
-    for e, (tl, vl) in enumerate(hist):
-        log(f"epoch {e}: train {tl:.5f} val {vl:.5f}")
+    hist = np.zeros((epochs, 2), np.float32)
+    for e in range(epochs):
+        lr_e = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * e / max(1, epochs))))
+        hist[e] = ctx.surrogate_train(1, batch=batch, lr=float(lr_e), n_val=n_val,
+                                      seed=seed + 1 + e)[0]
+        log(f"epoch {e}: lr {lr_e:.2e} train {hist[e, 0]:.5f} val {hist[e, 1]:.5f} "
+            f"({time.time() - t1:.0f}s)")
     return hist
 
 
@@ -62,7 +67,8 @@ def main():
     case = synth.batched_case(1)
     ctx = Context(0)
     ctx.set_model(1.0 / case["vel"], case["h"])
-    hist = train(ctx, n_models=a.models, spm=a.spm, epochs=a.epochs, batch=a.batch, lr=a.lr)
+    hist = train(ctx, n_models=a.models, spm=a.spm, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                 log=lambda m: print(m, flush=True))
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     ctx.surrogate_save(a.out)
     print(f"saved {a.out}")

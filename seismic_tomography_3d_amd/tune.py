@@ -18,8 +18,9 @@ CUS = 256  # MI355X compute units
 
 
 def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=64, log=None,
-                candidates=None):
-    key = (ctx.shape, len(src), block, grid, max_inner)
+                candidates=None, batch=None):
+    key = (ctx.shape, len(src), block, grid, max_inner, batch,
+           None if candidates is None else tuple(tuple(sorted(c.items())) for c in candidates))
     if key in _CACHE:
         return _CACHE[key]
     if block and grid and max_inner:
@@ -37,9 +38,12 @@ def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=64, log=None
                     candidates.append(dict(block=B, grid=wpc * CUS, max_inner=mi))
     best, best_ms = None, float("inf")
     for c in candidates:
-        ctx.fim_solve(sub[:2], rcv, batch=n, **c)  # first-touch of buffers
-        _, _, st = ctx.fim_solve(sub, rcv, batch=n, **c)
-        ms = st["solve_ms"]
+        c = dict(c)
+        b = c.pop("batch", batch or n)
+        ctx.fim_solve(sub[:2], rcv, batch=b, **c)  # first-touch of buffers
+        _, _, st = ctx.fim_solve(sub, rcv, batch=b, **c)
+        c["batch"] = b
+        ms = st["solve_ms"] if b > 1 else st["total_ms"]
         if log:
             log(f"tune {c}: {ms:.2f} ms for {n} shots, {st['visits']} visits")
         if ms < best_ms:
@@ -56,3 +60,25 @@ def load_surrogate(ctx, case, log=None):
     except ImportError:
         return False
     return surrogate.attach(ctx, case, log=log)
+
+
+def best_warm_params(ctx, src, rcv, cold, sample=64, log=None):
+    """Band of the predicted-arrival ordering (and inner cap) for warm solves."""
+    n = min(sample, len(src))
+    idx = np.linspace(0, len(src) - 1, n).round().astype(int)
+    sub = np.ascontiguousarray(src[idx])
+    h = ctx.spacing[0]
+    # one block crossing at 1500 m/s as the unit of the band
+    unit = cold["block"] * h / 1500.0
+    best, best_ms = None, float("inf")
+    for f in (0.5, 1.0, 2.0, 4.0):
+        o = dict(cold, batch=n, warm=1, band=f * unit)
+        ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
+        _, _, st = ctx.fim_solve(sub, rcv, **o)
+        ms = st["solve_ms"]
+        if log:
+            log(f"tune warm band {f * unit:.3f}s: {ms:.2f} ms for {n} shots, {st['visits']} visits, "
+                f"{st['fallbacks']} fallbacks")
+        if ms < best_ms:
+            best, best_ms = dict(band=f * unit), ms
+    return best
