@@ -40,7 +40,8 @@ typedef struct {
     float eps;          /* change threshold in seconds (<0 = default 0)        */
     int warm;           /* 0 cold start, 1 surrogate warm start                */
     float delta;        /* warm start inflation (1+delta) (0 = 0.02)           */
-    int judge;          /* 1: verify warm-started shots, re-solve cold on failure */
+    int judge;          /* 1: verify shots started from seeded values
+                           (warm_values = 1), re-solve cold on failure         */
     int grid;           /* workgroups per round launch (0 = auto)              */
     int rounds_per_sync;/* rounds enqueued between host checks (0 = auto)      */
     int use_graph;      /* 1: replay rounds from a captured hipGraph           */

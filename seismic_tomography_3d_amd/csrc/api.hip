@@ -259,7 +259,9 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
         }
         fim_batch(ctx, dsrc, nb, o, stats, warm, keys);
         std::vector<int> bad;
-        if (warm && o.judge) {
+        // the judge can only reject seeded values: an ordering-only warm start
+        // keeps the cold initial state and reaches the cold fixed point
+        if (warm && o.warm_values != 0 && o.judge) {
             judge_batch(ctx, dsrc, nb, B, res.data());
             for (int j = 0; j < nb; j++) {
                 if (stats) stats->judge_max = std::max(stats->judge_max, (double)res[j]);

@@ -538,14 +538,18 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
 // (source node excluded).  A converged solve from an upper bound has none:
 // every cell holds the update of its final neighbours.  A positive residual
 // marks cells whose warm value was below what their neighbours support.
+// One thread per padded cell; vol is a multiple of 64 (B^3 cells per block),
+// so every wave lies in one shot and a wave-level max leaves one atomic per
+// wave instead of one per cell.
 __global__ void judge_residual(const float *__restrict__ T, const float *__restrict__ S,
                                const float *__restrict__ src, int nshot, int B, int nbx, int nby,
                                int n4x, int n4y, int n4z, long long vol, float dx, float dy,
                                float dz, unsigned *__restrict__ resmax) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)nshot * vol) return;
-    const int s = (int)(i / vol);
-    const long long a = i - (long long)s * vol;
+    const long long n = (long long)nshot * vol;
+    const long long iw = i < n ? i : n - 1;  // tail lanes repeat the last cell
+    const int s = (int)(iw / vol);
+    const long long a = iw - (long long)s * vol;
     const int B3 = B * B * B;
     const long long b = a / B3;
     const int t = (int)(a - b * B3);
@@ -553,21 +557,23 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
     const long long rest = b / nbx;
     const int y = (int)(rest % nby) * B + (t / B) % B;
     const int z = (int)(rest / nby) * B + t / (B * B);
-    if (x >= n4x || y >= n4y || z >= n4z) return;
     const int ix = (int)(src[3 * s] / dx), iy = (int)(src[3 * s + 1] / dy), iz = (int)(src[3 * s + 2] / dz);
-    if (x == ix && y == iy && z == iz) return;
-    const float *Ts = T + (long long)s * vol;
-    auto at = [&](int xx, int yy, int zz) -> float {
-        if (xx < 0 || yy < 0 || zz < 0 || xx >= n4x || yy >= n4y || zz >= n4z) return kInf;
-        return Ts[blk_addr(xx, yy, zz, B, nbx, nby)];
-    };
-    const float v = Ts[a];
-    const float hs = dx * S[a];
-    const float g = godunov(fminf(at(x - 1, y, z), at(x + 1, y, z)),
-                            fminf(at(x, y - 1, z), at(x, y + 1, z)),
-                            fminf(at(x, y, z - 1), at(x, y, z + 1)), hs, hs * hs);
-    const float r = g - v;
-    if (r > 0.0f) atomicMax(&resmax[s], __float_as_uint(r));
+    float r = 0.0f;
+    if (x < n4x && y < n4y && z < n4z && !(x == ix && y == iy && z == iz)) {
+        const float *Ts = T + (long long)s * vol;
+        auto at = [&](int xx, int yy, int zz) -> float {
+            if (xx < 0 || yy < 0 || zz < 0 || xx >= n4x || yy >= n4y || zz >= n4z) return kInf;
+            return Ts[blk_addr(xx, yy, zz, B, nbx, nby)];
+        };
+        const float v = Ts[a];
+        const float hs = dx * S[a];
+        const float g = godunov(fminf(at(x - 1, y, z), at(x + 1, y, z)),
+                                fminf(at(x, y - 1, z), at(x, y + 1, z)),
+                                fminf(at(x, y, z - 1), at(x, y, z + 1)), hs, hs * hs);
+        r = fmaxf(g - v, 0.0f);
+    }
+    for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
+    if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&resmax[s], __float_as_uint(r));
 }
 }  // namespace st
 

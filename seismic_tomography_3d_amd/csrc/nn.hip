@@ -98,6 +98,119 @@ void gemm(hipStream_t st, const GemmArgs &g) {
 }
 
 // ---------------------------------------------------------------------------
+// Implicit-GEMM 3D convolution, forward (stride 1, zero "same" padding):
+//   Y[s, v, n] = act(b[n] + sum_{tap, c} X[s, v + off(tap), c] * W[tap*cin + c, n])
+// One workgroup (4 waves) per 4x4x4 output brick of one sample.  The
+// (4 + KS - 1)^3 input halo, all channels, is staged once in LDS as BF16
+// (row stride cinp + 4 halves to spread LDS banks); wave w owns the 16
+// voxels of brick slab z = w and all output channels, so each A fragment
+// read from LDS feeds NT v_mfma_f32_16x16x16_bf16.  K = tap*cinp + c runs
+// in steps of 16: a lane's four consecutive k share one tap (cinp % 4 == 0),
+// so its A fragment is one 8-byte LDS read and its B fragment one 8-byte
+// load from the packed weights Wp[tap][n][c] (n padded to 16*NT, c to
+// cinp, an extra all-zero tap absorbing the K padding).
+// ---------------------------------------------------------------------------
+template <int KS, int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X,
+                                                  const unsigned short *__restrict__ Wp,
+                                                  const float *__restrict__ bias,
+                                                  float *__restrict__ Y, int e, int cin, int cinp,
+                                                  int cout, int relu) {
+    constexpr int H = 4 + KS - 1, H3 = H * H * H, TAPS = KS * KS * KS, HF = KS / 2;
+    extern __shared__ unsigned short halo[];
+    const int cst = cinp + 4;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nb = (e + 3) >> 2;
+    const int bid = blockIdx.x;
+    const int bx = bid % nb, by = (bid / nb) % nb, bz = (bid / (nb * nb)) % nb;
+    const int s = bid / (nb * nb * nb);
+    const long long V = (long long)e * e * e;
+    const float *Xs = X + (long long)s * V * cin;
+    const int ox = 4 * bx - HF, oy = 4 * by - HF, oz = 4 * bz - HF;
+
+    // stage the halo: 4 channels per item
+    const int c4 = cinp >> 2;
+    for (int i = tid; i < H3 * c4; i += 256) {
+        const int hv = i / c4, c = (i - hv * c4) * 4;
+        const int hx = hv % H, hy = (hv / H) % H, hz = hv / (H * H);
+        const int xx = ox + hx, yy = oy + hy, zz = oz + hz;
+        float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c < cin && xx >= 0 && yy >= 0 && zz >= 0 && xx < e && yy < e && zz < e) {
+            const float *p = Xs + (xx + (long long)e * (yy + (long long)e * zz)) * cin + c;
+            if ((cin & 3) == 0) {
+                f = *reinterpret_cast<const float4 *>(p);
+            } else {
+                if (c + 0 < cin) f.x = p[0];
+                if (c + 1 < cin) f.y = p[1];
+                if (c + 2 < cin) f.z = p[2];
+                if (c + 3 < cin) f.w = p[3];
+            }
+        }
+        uint2 u;
+        u.x = (unsigned)f2bf(f.x) | ((unsigned)f2bf(f.y) << 16);
+        u.y = (unsigned)f2bf(f.z) | ((unsigned)f2bf(f.w) << 16);
+        *reinterpret_cast<uint2 *>(&halo[hv * cst + c]) = u;
+    }
+    __syncthreads();
+
+    const int m = lane & 15, q = lane >> 4;
+    const int vx = m & 3, vy = m >> 2, vz = w;
+    const int coutp = 16 * NT;
+    f4v acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[j] = f4v{0.f, 0.f, 0.f, 0.f};
+    const int K16 = (TAPS * cinp + 15) >> 4;
+    int tap = (4 * q) / cinp, c = 4 * q - tap * cinp;
+    for (int ks = 0; ks < K16; ks++) {
+        const int tp = tap < TAPS ? tap : 0;  // padding tap: zero weights
+        const int tx = tp % KS, ty = (tp / KS) % KS, tz = tp / (KS * KS);
+        const int hv = (vx + tx) + H * ((vy + ty) + H * (vz + tz));
+        const s4v a = *reinterpret_cast<const s4v *>(&halo[hv * cst + c]);
+        const unsigned short *wb = Wp + ((long long)tap * coutp + m) * cinp + c;
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const s4v b = *reinterpret_cast<const s4v *>(wb + (long long)j * 16 * cinp);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+        c += 16;
+        while (c >= cinp) { c -= cinp; tap++; }
+    }
+
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const int n = j * 16 + m;
+        if (n >= cout) continue;
+        const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int mo = 4 * q + r;
+            const int x = 4 * bx + (mo & 3), y = 4 * by + (mo >> 2), z = 4 * bz + vz;
+            if (x >= e || y >= e || z >= e) continue;
+            float v = acc[j][r] + bn;
+            if (relu) v = fmaxf(v, 0.f);
+            Y[((long long)s * V + x + (long long)e * (y + (long long)e * z)) * cout + n] = v;
+        }
+    }
+}
+
+// Wp[tap][n][c] (taps + 1 taps, n < 16*ceil(cout/16), c < cinp) from
+// W16[tap*cin + c][cout]; zero where tap == taps, n >= cout or c >= cin.
+__global__ void pack_conv(const unsigned short *__restrict__ W16, unsigned short *__restrict__ Wp,
+                          int taps, int cin, int cout, int cinp, int coutp) {
+    const long long tot = (long long)(taps + 1) * coutp * cinp;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % cinp);
+        const long long r = i / cinp;
+        const int n = (int)(r % coutp);
+        const int tap = (int)(r / coutp);
+        unsigned short v = 0;
+        if (tap < taps && n < cout && c < cin) v = W16[((long long)tap * cin + c) * cout + n];
+        Wp[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // elementwise / layout kernels
 // ---------------------------------------------------------------------------
 #define GRID_STRIDE(i, n) \
@@ -362,6 +475,47 @@ void adam_step(hipStream_t st, float *w, const float *g, float *m, float *v, uns
 }
 void bf16_copy(hipStream_t st, const float *x, unsigned short *y, long long n) {
     hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, st, x, y, n);
+}
+
+
+int conv_cinp(int cin) { return (cin + 7) / 8 * 8; }
+int conv_coutp(int cout) { return cout <= 16 ? 16 : (cout <= 32 ? 32 : 64); }
+long long conv_packed_size(int ksz, int cin, int cout) {
+    return (long long)(ksz * ksz * ksz + 1) * conv_coutp(cout) * conv_cinp(cin);
+}
+
+void pack_conv_weights(hipStream_t st, const unsigned short *W16, unsigned short *Wp, int ksz,
+                       int cin, int cout) {
+    const int taps = ksz * ksz * ksz;
+    const long long tot = conv_packed_size(ksz, cin, cout);
+    hipLaunchKernelGGL(pack_conv, dim3(blocks_for(tot)), dim3(256), 0, st, W16, Wp, taps, cin,
+                       cout, conv_cinp(cin), conv_coutp(cout));
+    ST_HIP(hipGetLastError());
+}
+
+void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, const float *bias,
+                  float *Y, int ns, int e, int cin, int cout, int ksz, bool relu) {
+    ST_REQUIRE(ksz == 1 || ksz == 3, ST_ERR_ARG, "conv kernel size must be 1 or 3");
+    ST_REQUIRE(cout >= 1 && cout <= 64 && cin >= 1 && cin <= 256 && e >= 1, ST_ERR_ARG,
+               "conv shape out of range");
+    const int cinp = conv_cinp(cin), coutp = conv_coutp(cout);
+    const int H = 4 + ksz - 1;
+    const size_t lds = sizeof(unsigned short) * (size_t)H * H * H * (cinp + 4);
+    ST_REQUIRE(lds <= 65536, ST_ERR_ARG, "conv input halo exceeds 64 KiB of LDS");
+    const int nb = (e + 3) / 4;
+    const long long nwg = (long long)ns * nb * nb * nb;
+    ST_REQUIRE(nwg < (1LL << 31), ST_ERR_ARG, "conv grid too large");
+    if (nwg == 0) return;
+    const dim3 g((unsigned)nwg), b(256);
+    const int r = relu ? 1 : 0;
+#define ST_CONV(KS, NT) hipLaunchKernelGGL((conv3d_fwd<KS, NT>), g, b, lds, st, X, Wp, bias, Y, e, cin, cinp, cout, r)
+    if (ksz == 3) {
+        if (coutp == 16) ST_CONV(3, 1); else if (coutp == 32) ST_CONV(3, 2); else ST_CONV(3, 4);
+    } else {
+        if (coutp == 16) ST_CONV(1, 1); else if (coutp == 32) ST_CONV(1, 2); else ST_CONV(1, 4);
+    }
+#undef ST_CONV
+    ST_HIP(hipGetLastError());
 }
 
 }  // namespace nn

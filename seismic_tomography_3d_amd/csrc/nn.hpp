@@ -47,5 +47,14 @@ void adam_step(hipStream_t st, float *w, const float *g, float *m, float *v, uns
                long long n, float lr, int step);
 void bf16_copy(hipStream_t st, const float *x, unsigned short *y, long long n);
 
+// implicit-GEMM convolution forward (conv3d_fwd) and its weight packing
+int conv_cinp(int cin);
+int conv_coutp(int cout);
+long long conv_packed_size(int ksz, int cin, int cout);
+void pack_conv_weights(hipStream_t st, const unsigned short *W16, unsigned short *Wp, int ksz,
+                       int cin, int cout);
+void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, const float *bias,
+                  float *Y, int ns, int e, int cin, int cout, int ksz, bool relu);
+
 }  // namespace nn
 }  // namespace st

@@ -41,6 +41,7 @@ struct Conv {
     int cin, cout, ksz, res_div;  // res_div: cube edge = nc / res_div
     bool relu;
     long long ow, ob;             // offsets of W [taps*cin, cout] and b [cout]
+    long long op;                 // offset of the packed BF16 weights (conv3d_fwd)
 };
 
 // conv index -> (cin, cout, ksz, resolution divisor, relu)
@@ -61,12 +62,14 @@ struct Net {
     int nc = 0;
     std::vector<Conv> L;
     long long nparam = 0;
-    DevBuf w, g, m, v, w16;
+    DevBuf w, g, m, v, w16, wp;
+    long long npacked = 0;
+    unsigned wver = 1, pver = 0;  // weight version / version the packing holds
     int step = 0;
     bool trained = false;
     float dmin = 0.f, rmask = 0.f;  // metres, set from the training grid
-    // activation / gradient buffers for the current batch capacity
-    int cap = 0;
+    // activation / gradient buffers for the current batch capacities
+    int cap = 0, tcap = 0;
     DevBuf in0, a1, a2, p1, a3, a4, p2, a5, p3, a7, cat3, a8, cat2, a9, cat1, a10, out;
     DevBuf d_out, d_a10, d_cat1, d_a9, d_a2, d_t32, d_cat2, d_a8, d_a4, d_t16, d_cat3, d_a7, d_a5,
         d_t8, d_p3, d_p2, d_a3, d_p1, d_a1, d_in;
@@ -90,7 +93,7 @@ void surrogate_release(st_ctx *ctx) {
     if (!ctx->surr) return;
     SurrState &s = surr(ctx);
     Net &n = s.net;
-    for (DevBuf *b : {&n.w, &n.g, &n.m, &n.v, &n.w16, &n.in0, &n.a1, &n.a2, &n.p1, &n.a3, &n.a4,
+    for (DevBuf *b : {&n.w, &n.g, &n.m, &n.v, &n.w16, &n.wp, &n.in0, &n.a1, &n.a2, &n.p1, &n.a3, &n.a4,
                       &n.p2, &n.a5, &n.p3, &n.a7, &n.cat3, &n.a8, &n.cat2, &n.a9, &n.cat1, &n.a10,
                       &n.out, &n.d_out, &n.d_a10, &n.d_cat1, &n.d_a9, &n.d_a2, &n.d_t32, &n.d_cat2,
                       &n.d_a8, &n.d_a4, &n.d_t16, &n.d_cat3, &n.d_a7, &n.d_a5, &n.d_t8, &n.d_p3,
@@ -105,23 +108,28 @@ static void net_define(Net &n, int nc) {
     ST_REQUIRE(nc >= 8 && (nc & (nc - 1)) == 0, ST_ERR_ARG, "node cube edge must be a power of 2 >= 8");
     n.nc = nc;
     n.L.clear();
-    long long off = 0;
+    long long off = 0, poff = 0;
     for (auto &a : kArch) {
-        Conv c{a[0], a[1], a[2], a[3], a[4] != 0, 0, 0};
+        Conv c{a[0], a[1], a[2], a[3], a[4] != 0, 0, 0, 0};
         const int taps = c.ksz * c.ksz * c.ksz;
         c.ow = off;
         off += (long long)taps * c.cin * c.cout;
         c.ob = off;
         off += c.cout;
+        c.op = poff;
+        poff += (nn::conv_packed_size(c.ksz, c.cin, c.cout) + 127) / 128 * 128;
         n.L.push_back(c);
     }
     n.nparam = off;
+    n.npacked = poff;
 }
 
 static void net_alloc_params(st_ctx *ctx, Net &n, unsigned seed) {
     const size_t b = sizeof(float) * n.nparam;
     n.w.reserve(b); n.g.reserve(b); n.m.reserve(b); n.v.reserve(b);
     n.w16.reserve(sizeof(unsigned short) * n.nparam);
+    n.wp.reserve(sizeof(unsigned short) * n.npacked);
+    n.wver++;
     std::vector<float> h(n.nparam, 0.f);
     std::mt19937_64 rng(seed);
     for (auto &c : n.L) {
@@ -135,21 +143,27 @@ static void net_alloc_params(st_ctx *ctx, Net &n, unsigned seed) {
     ST_HIP(hipMemsetAsync(n.m.ptr, 0, b, ctx->stream));
     ST_HIP(hipMemsetAsync(n.v.ptr, 0, b, ctx->stream));
     nn::bf16_copy(ctx->stream, n.w.as<float>(), n.w16.as<unsigned short>(), n.nparam);
+    n.wver++;
     ST_HIP(hipStreamSynchronize(ctx->stream));
     n.step = 0;
 }
 
 __host__ __device__ inline long long cube(int e) { return (long long)e * e * e; }
 
-static void net_alloc_batch(Net &n, int ns) {
-    if (ns <= n.cap) return;
+// forward buffers for ns samples; with train also the gradient and im2col
+// buffers of the backward pass
+static void net_alloc_batch(Net &n, int ns, bool train) {
     const int e1 = n.nc, e2 = n.nc / 2, e4 = n.nc / 4, e8 = n.nc / 8;
     const long long v1 = cube(e1) * ns, v2 = cube(e2) * ns, v4 = cube(e4) * ns, v8 = cube(e8) * ns;
     auto F = [](DevBuf &b, long long n) { b.reserve(sizeof(float) * std::max(n, 1LL)); };
-    F(n.in0, v1 * kIn); F(n.a1, v1 * 16); F(n.a2, v1 * 16); F(n.p1, v2 * 16); F(n.a3, v2 * 32);
-    F(n.a4, v2 * 32); F(n.p2, v4 * 32); F(n.a5, v4 * 64); F(n.p3, v8 * 64); F(n.a7, v8 * 64);
-    F(n.cat3, v4 * 128); F(n.a8, v4 * 64); F(n.cat2, v2 * 96); F(n.a9, v2 * 32);
-    F(n.cat1, v1 * 48); F(n.a10, v1 * 16); F(n.out, v1);
+    if (ns > n.cap) {
+        F(n.in0, v1 * kIn); F(n.a1, v1 * 16); F(n.a2, v1 * 16); F(n.p1, v2 * 16); F(n.a3, v2 * 32);
+        F(n.a4, v2 * 32); F(n.p2, v4 * 32); F(n.a5, v4 * 64); F(n.p3, v8 * 64); F(n.a7, v8 * 64);
+        F(n.cat3, v4 * 128); F(n.a8, v4 * 64); F(n.cat2, v2 * 96); F(n.a9, v2 * 32);
+        F(n.cat1, v1 * 48); F(n.a10, v1 * 16); F(n.out, v1);
+        n.cap = ns;
+    }
+    if (!train || ns <= n.tcap) return;
     F(n.d_out, v1); F(n.d_a10, v1 * 16); F(n.d_cat1, v1 * 48); F(n.d_a9, v2 * 32);
     F(n.d_a2, v1 * 16); F(n.d_t32, v1 * 16); F(n.d_cat2, v2 * 96); F(n.d_a8, v4 * 64);
     F(n.d_a4, v2 * 32); F(n.d_t16, v2 * 32); F(n.d_cat3, v4 * 128); F(n.d_a7, v8 * 64);
@@ -165,27 +179,23 @@ static void net_alloc_batch(Net &n, int ns) {
     n.dcol.reserve(sizeof(float) * colmax);
     n.d16.reserve(sizeof(unsigned short) * ymax);
     n.lossacc.reserve(64);
-    n.cap = ns;
+    n.tcap = ns;
 }
 
-// one convolution forward: Y = act(conv(X) + b)
+// one convolution forward: Y = act(conv(X) + b), implicit GEMM on MFMA
 static void conv_fwd(st_ctx *ctx, Net &n, int li, const float *X, float *Y, int ns) {
     const Conv &c = n.L[li];
-    const int e = n.nc / c.res_div;
-    const int taps = c.ksz * c.ksz * c.ksz;
-    nn::im2col(ctx->stream, X, n.col.as<unsigned short>(), ns, e, c.cin, c.ksz);
-    nn::GemmArgs g{};
-    g.A = n.col.as<unsigned short>();
-    g.B = n.w16.as<unsigned short>() + c.ow;
-    g.C = Y;
-    g.bias = n.w.as<float>() + c.ob;
-    g.M = cube(e) * ns;
-    g.N = c.cout;
-    g.K = taps * c.cin;
-    g.lda = g.K; g.ldb = c.cout; g.ldc = c.cout;
-    g.relu = c.relu;
-    g.ksplit = 1;
-    nn::gemm(ctx->stream, g);
+    nn::conv_forward(ctx->stream, X, n.wp.as<unsigned short>() + c.op, n.w.as<float>() + c.ob, Y,
+                     ns, n.nc / c.res_div, c.cin, c.cout, c.ksz, c.relu);
+}
+
+// re-packs the BF16 weights for conv3d_fwd after any change of n.w16
+static void net_pack(st_ctx *ctx, Net &n) {
+    if (n.pver == n.wver) return;
+    for (auto &c : n.L)
+        nn::pack_conv_weights(ctx->stream, n.w16.as<unsigned short>() + c.ow,
+                              n.wp.as<unsigned short>() + c.op, c.ksz, c.cin, c.cout);
+    n.pver = n.wver;
 }
 
 // one convolution backward: dY (in place masked) -> dW, db (+=), dX (written)
@@ -236,6 +246,7 @@ static void add(st_ctx *ctx, float *y, const float *x, long long n) {
 static void net_forward(st_ctx *ctx, Net &n, int ns) {
     hipStream_t st = ctx->stream;
     const int e1 = n.nc, e2 = e1 / 2, e4 = e1 / 4, e8 = e1 / 8;
+    net_pack(ctx, n);
     conv_fwd(ctx, n, 0, n.in0.as<float>(), n.a1.as<float>(), ns);
     conv_fwd(ctx, n, 1, n.a1.as<float>(), n.a2.as<float>(), ns);
     nn::pool(st, n.a2.as<float>(), n.p1.as<float>(), ns, e1, 16);
@@ -463,8 +474,8 @@ bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float
     const long long V = cube(n.nc);
     const CubeGeo g = cube_geo(ctx, n.nc);
     n.Q.reserve(sizeof(float) * V * nshot);
-    const int chunk = 32;
-    net_alloc_batch(n, std::min(chunk, nshot));
+    const int chunk = 128;
+    net_alloc_batch(n, std::min(chunk, nshot), false);
     for (int s0 = 0; s0 < nshot; s0 += chunk) {
         const int ns = std::min(chunk, nshot - s0);
         hipLaunchKernelGGL(prep_inputs, dim3(nblocks(V * ns)), dim3(256), 0, ctx->stream,
@@ -511,6 +522,7 @@ int st_surrogate_init(st_ctx *ctx, int nc, unsigned seed) {
         net_alloc_params(ctx, n, seed);
         n.trained = false;
         n.cap = 0;
+        n.tcap = 0;
     });
 }
 
@@ -532,6 +544,7 @@ int st_surrogate_params(st_ctx *ctx, float *get, const float *set) {
         if (set) {
             ST_HIP(hipMemcpyAsync(n.w.ptr, set, b, hipMemcpyHostToDevice, ctx->stream));
             nn::bf16_copy(ctx->stream, n.w.as<float>(), n.w16.as<unsigned short>(), n.nparam);
+            n.wver++;
         }
         if (get) ST_HIP(hipMemcpyAsync(get, n.w.ptr, b, hipMemcpyDeviceToHost, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
@@ -548,7 +561,7 @@ int st_surrogate_eval(st_ctx *ctx, const float *X, int ns, const float *target, 
         ST_REQUIRE(X && ns > 0, ST_ERR_ARG, "bad inputs");
         ST_HIP(hipSetDevice(ctx->device));
         Net &n = surr(ctx).net;
-        net_alloc_batch(n, ns);
+        net_alloc_batch(n, ns, target && mask);
         const long long V = cube(n.nc) * ns;
         ST_HIP(hipMemcpyAsync(n.in0.ptr, X, sizeof(float) * V * kIn, hipMemcpyHostToDevice, ctx->stream));
         net_forward(ctx, n, ns);
@@ -686,7 +699,7 @@ int st_surrogate_train(st_ctx *ctx, int epochs, int batch, float lr, int n_val, 
         ST_HIP(hipSetDevice(ctx->device));
         const long long V = cube(n.nc);
         const long long ntrain = n.nsamples - n_val;
-        net_alloc_batch(n, batch);
+        net_alloc_batch(n, batch, true);
         n.bX.reserve(sizeof(float) * V * kIn * batch);
         n.bY.reserve(sizeof(float) * V * batch);
         n.bM.reserve(sizeof(float) * V * batch);
@@ -706,6 +719,7 @@ int st_surrogate_train(st_ctx *ctx, int epochs, int batch, float lr, int n_val, 
                 n.step++;
                 nn::adam_step(ctx->stream, n.w.as<float>(), n.g.as<float>(), n.m.as<float>(),
                               n.v.as<float>(), n.w16.as<unsigned short>(), n.nparam, lr, n.step);
+                n.wver++;
             }
             double h[2];
             ST_HIP(hipMemcpyAsync(h, n.lossacc.ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -780,11 +794,13 @@ int st_surrogate_load(st_ctx *ctx, const char *path) {
         net_alloc_params(ctx, n, 0);
         ST_HIP(hipMemcpy(n.w.ptr, h.data(), sizeof(float) * n.nparam, hipMemcpyHostToDevice));
         nn::bf16_copy(ctx->stream, n.w.as<float>(), n.w16.as<unsigned short>(), n.nparam);
+        n.wver++;
         ST_HIP(hipStreamSynchronize(ctx->stream));
         n.dmin = geo[0];
         n.rmask = geo[1];
         n.trained = true;
         n.cap = 0;
+        n.tcap = 0;
     });
 }
 
@@ -799,7 +815,7 @@ int st_surrogate_predict(st_ctx *ctx, const float *src_xyz, int nshot, float *y_
         ST_HIP(hipSetDevice(ctx->device));
         Net &n = surr(ctx).net;
         const long long V = cube(n.nc);
-        net_alloc_batch(n, nshot);
+        net_alloc_batch(n, nshot, false);
         ctx->src.reserve(sizeof(float) * 3 * nshot);
         ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot, hipMemcpyHostToDevice,
                               ctx->stream));

@@ -33,8 +33,8 @@ def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=64, log=None
     if candidates is None:
         candidates = []
         for B in ([block] if block else [4, 8]):
-            for wpc in ([grid // CUS] if grid else [8, 16, 32]):
-                for mi in ([max_inner] if max_inner else ([12, 16] if B == 4 else [6, 8])):
+            for wpc in ([grid // CUS] if grid else ([8] if B == 4 else [8, 16, 32])):
+                for mi in ([max_inner] if max_inner else ([16] if B == 4 else [16, 24])):
                     candidates.append(dict(block=B, grid=wpc * CUS, max_inner=mi))
     best, best_ms = None, float("inf")
     for c in candidates:
@@ -63,7 +63,8 @@ def load_surrogate(ctx, case, log=None):
 
 
 def best_warm_params(ctx, src, rcv, cold, sample=64, log=None):
-    """Band of the predicted-arrival ordering (and inner cap) for warm solves."""
+    """Band of the predicted-arrival ordering and inner cap for warm solves,
+    timed over the whole call (network inference included)."""
     n = min(sample, len(src))
     idx = np.linspace(0, len(src) - 1, n).round().astype(int)
     sub = np.ascontiguousarray(src[idx])
@@ -71,14 +72,16 @@ def best_warm_params(ctx, src, rcv, cold, sample=64, log=None):
     # one block crossing at 1500 m/s as the unit of the band
     unit = cold["block"] * h / 1500.0
     best, best_ms = None, float("inf")
-    for f in (0.5, 1.0, 2.0, 4.0):
-        o = dict(cold, batch=n, warm=1, band=f * unit)
-        ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
-        _, _, st = ctx.fim_solve(sub, rcv, **o)
-        ms = st["solve_ms"]
-        if log:
-            log(f"tune warm band {f * unit:.3f}s: {ms:.2f} ms for {n} shots, {st['visits']} visits, "
-                f"{st['fallbacks']} fallbacks")
-        if ms < best_ms:
-            best, best_ms = dict(band=f * unit), ms
+    for mi in sorted({cold["max_inner"], 24}):
+        for f in (1.0, 1.5, 2.0, 3.0):
+            o = dict(cold, batch=n, warm=1, band=f * unit, max_inner=mi)
+            ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
+            _, _, st = ctx.fim_solve(sub, rcv, **o)
+            ms = st["total_ms"]
+            if log:
+                log(f"tune warm max_inner {mi} band {f * unit:.3f}s: {ms:.2f} ms "
+                    f"(solve {st['solve_ms']:.2f}) for {n} shots, {st['visits']} visits, "
+                    f"{st['fallbacks']} fallbacks")
+            if ms < best_ms:
+                best, best_ms = dict(band=f * unit, max_inner=mi), ms
     return best

@@ -129,3 +129,29 @@ def test_graph_replay_matches_eager(ctx):
         ctx.fim_solve(srcs[:2], rcv, batch=2, use_graph=0)  # eager work between replays
     pg1, _, _ = ctx.fim_solve(srcs[:1], rcv, batch=1, use_graph=1)  # new shape: recapture
     assert np.abs(pg1[0] - pe[0]).max() < 1e-5
+
+
+def test_warm_values_judge_and_fallback(ctx):
+    """Seeded (1+delta) values: an upper-bound prediction passes the judge;
+    an under-estimate (half the true times) is not a valid start for the
+    monotone solver, is caught by the residual judge and re-solved cold.
+    Both end at the FP64 fixed point within the metric's tolerance."""
+    vel, h = _case(seed=5)
+    ctx.set_model(1.0 / vel, h)
+    srcs = synth.random_sources(3, 41, 37, h, 0.0, 400.0, seed=31)
+    rcv = np.stack(synth.surface_grid(5, 25.0, 875.0, 0.0), axis=1)
+    T64 = np.stack([orc.eikonal_fp64(vel, h, s) for s in srcs]).astype(np.float32)
+    p_ok, T_ok, st_ok = ctx.fim_solve_warm(srcs, rcv, T64, return_T=True, batch=3,
+                                           warm_values=1, delta=0.02)
+    print("upper bound:", st_ok)
+    assert st_ok["fallbacks"] == 0
+    assert np.abs(T_ok - T64).max() < TOL
+    p_bad, T_bad, st_bad = ctx.fim_solve_warm(srcs, rcv, 0.5 * T64, return_T=True, batch=3,
+                                              warm_values=1, delta=0.02)
+    print("under-estimate:", st_bad)
+    assert st_bad["fallbacks"] == 3 and st_bad["judge_max"] > 2e-6
+    assert np.abs(T_bad - T64).max() < TOL
+    # ordering only: same fixed point, no judge needed
+    p_ord, T_ord, st_ord = ctx.fim_solve_warm(srcs, rcv, 0.5 * T64, return_T=True, batch=3)
+    assert st_ord["fallbacks"] == 0
+    assert np.abs(T_ord - T64).max() < TOL

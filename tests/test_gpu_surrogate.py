@@ -38,8 +38,11 @@ def _inputs(nc, ns, seed):
     return X, T, M
 
 
-def test_network_forward_matches_fp64(ctx):
-    nc, ns = 8, 2
+@pytest.mark.parametrize("nc,ns", [(8, 2), (16, 3)])
+def test_network_forward_matches_fp64(ctx, nc, ns):
+    """Implicit-GEMM convolutions (BF16 operands, FP32 accumulation) against
+    the FP64 numpy network; nc = 8 and 16 put 1^3 and 2^3 cubes (partial
+    4x4x4 bricks) at the bottleneck."""
     ctx.surrogate_init(nc, seed=3)
     p = ctx.surrogate_params()
     # perturb the head so the output depends on every layer
