@@ -60,30 +60,32 @@ constexpr unsigned kBlkMask = (1u << kShotShift) - 1u;
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // First-order Godunov upwind update from the smallest neighbour of each axis
-// (a, b, c), hs = h*s.  Same branch structure as get_time_eikonal
-// (cuda_kernel.cu:217-248); the 3D discriminant is written as
-// 3(hs)^2 - [(a-b)^2 + (b-c)^2 + (c-a)^2], which equals the reference's
-// 2(a(b-a)+b(c-b)+c(a-c)) + 3(hs)^2 without its cancellation at large T.
+// (a, b, c), hs = h*s: the 1D candidate lo + hs while it does not exceed the
+// middle neighbour, else the 2D solution while it does not exceed the
+// largest, else the 3D solution -- the branch structure of get_time_eikonal
+// (cuda_kernel.cu:217-248), evaluated without branches (all three
+// candidates, then two selects) so that a wave issues no exec-mask
+// sequences.  The 3D discriminant is 3(hs)^2 - [(a-b)^2 + (b-c)^2 + (c-a)^2]
+// in sorted differences d = mid - lo, e1 = hi - mid, e3 = hi - lo = d + e1:
+// the bracket equals 2 (d e3 + e1^2), with no cancellation at large T (the
+// reference's expanded 2(a(b-a)+b(c-b)+c(a-c)) + 3(hs)^2 has it).  Square
+// roots take max(., 0): a negative argument belongs to a candidate that is
+// not selected, so no NaN is ever formed.  Unreached neighbours are the
+// finite sentinel kInf; a cell whose neighbours are all kInf gets a value
+// above kInf, which min(T, .) discards.
 __device__ __forceinline__ float godunov(float a, float b, float c, float hs, float hs2) {
-    const float hi = fmaxf(fmaxf(a, b), c);
     const float lo = fminf(fminf(a, b), c);
     const float mid = __builtin_amdgcn_fmed3f(a, b, c);
-    float ret = kInf;
-    if (lo < kInf) {
-        ret = lo + hs;
-        if (ret > mid) {
-            const float d = mid - lo;
-            const float t2 = 0.5f * ((mid + lo) + fsqrt(2.0f * hs2 - d * d));
-            if (t2 > mid) ret = t2;
-            if (ret > hi) {
-                const float e1 = hi - mid, e2 = mid - lo, e3 = hi - lo;
-                const float q = 3.0f * hs2 - (e1 * e1 + e2 * e2 + e3 * e3);
-                const float t3 = ((hi + mid + lo) + fsqrt(q)) * (1.0f / 3.0f);
-                if (t3 > hi) ret = t3;
-            }
-        }
-    }
-    return ret;
+    const float hi = fmaxf(fmaxf(a, b), c);
+    const float t1 = lo + hs;
+    const float d = mid - lo;
+    const float sm = mid + lo;
+    const float t2 = 0.5f * (sm + fsqrt(fmaxf(2.0f * hs2 - d * d, 0.0f)));
+    const float e1 = hi - mid, e3 = hi - lo;
+    const float u = d * e3 + e1 * e1;
+    const float t3 = ((hi + sm) + fsqrt(fmaxf(3.0f * hs2 - 2.0f * u, 0.0f))) * (1.0f / 3.0f);
+    const float r = t1 > mid ? t2 : t1;
+    return r > hi ? t3 : r;
 }
 
 // Schedules the candidates held by lanes 0..6 for the next round and
@@ -107,6 +109,60 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
         if (fresh) {
             const unsigned rank = (unsigned)__popcll(fm & ((1ull << lane) - 1ull));
             nxt[base + rank] = (shot << kShotShift) | target;
+        }
+    }
+}
+
+// Walks this workgroup's items of the current list (i = blockIdx.x + j *
+// gridDim.x, the same interleaving for every round) 64 at a time: lane l
+// reads item j0 + l.  With arrival ordering, the items whose block is not
+// due yet (predicted arrival above its shot's earliest scheduled key plus
+// the band) move to the next list right here, all lanes at once with one
+// append; the others are visited one after another by the whole wave.
+template <class Visit>
+__device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, const unsigned *cur,
+                                          unsigned *nxt, unsigned *nxt_cnt, unsigned bit_cur,
+                                          unsigned bit_nxt, int lane,
+                                          unsigned long long &my_defer, Visit visit) {
+    const unsigned long long G = gridDim.x;
+    for (unsigned long long j0 = 0; blockIdx.x + j0 * G < count; j0 += 64) {
+        const unsigned long long i = blockIdx.x + (j0 + lane) * G;
+        const bool in = i < count;
+        const unsigned item = in ? cur[i] : 0u;
+        bool run = in;
+        if (A.keys) {
+            bool fresh = false;
+            unsigned shot = 0;
+            if (in) {
+                shot = item >> kShotShift;
+                const unsigned b = item & kBlkMask;
+                const float tau =
+                    __uint_as_float(A.kmin[(A.round % 3) * A.nshot + shot]) + A.band;
+                const float key = A.keys[(long long)shot * A.nblk + b];
+                if (key > tau) {  // not its time yet: carried to the next round
+                    run = false;
+                    unsigned *fl = A.flags + (long long)shot * A.nblk;
+                    atomicAnd(&fl[b], ~bit_cur);
+                    fresh = !(atomicOr(&fl[b], bit_nxt) & bit_nxt);
+                    if (fresh)
+                        atomicMin(&A.kmin[((A.round + 1) % 3) * A.nshot + shot],
+                                  __float_as_uint(key));
+                }
+            }
+            const unsigned long long fm = __ballot(fresh);
+            if (fm) {
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(nxt_cnt, (unsigned)__popcll(fm));
+                base = __shfl(base, 0);
+                if (fresh) nxt[base + (unsigned)__popcll(fm & ((1ull << lane) - 1ull))] = item;
+            }
+            my_defer += (unsigned long long)__popcll(__ballot(in && !run));
+        }
+        unsigned long long m = __ballot(run);
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1ull;
+            visit((unsigned)__builtin_amdgcn_readlane((int)item, j));
         }
     }
 }
@@ -138,20 +194,11 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
     const int nbxy = A.nbx * A.nby;
     unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
-    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
-        const unsigned item = cur[i];
+    for_items(A, count, cur, nxt, nxt_cnt, bit_cur, bit_nxt, lane, my_defer, [&](const unsigned item) {
         const unsigned shot = item >> kShotShift, b = item & kBlkMask;
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
         if (lane == 0) atomicAnd(&fl[b], ~bit_cur);
-        if (A.keys) {
-            const float tau = __uint_as_float(A.kmin[(A.round % 3) * A.nshot + shot]) + A.band;
-            if (A.keys[(long long)shot * A.nblk + b] > tau) {  // not its time yet
-                schedule(A, shot, fl, lane == 0 ? b : 0xFFFFFFFFu, bit_nxt, nxt, nxt_cnt, lane);
-                my_defer += 1;
-                continue;
-            }
-        }
         const int bx = b % A.nbx, rest = b / A.nbx, by = rest % A.nby, bz = rest / A.nby;
         const long long base = (long long)b * 64;
 
@@ -229,7 +276,7 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
         my_visits += 1;
         my_iters += it;
         __syncthreads();
-    }
+    });
     if (lane == 0 && A.stats) {
         atomicAdd(&A.stats[0], my_visits);
         atomicAdd(&A.stats[1], my_iters);
@@ -266,20 +313,11 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
     const int nbxy = A.nbx * A.nby;
     unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
-    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
-        const unsigned item = cur[i];
+    for_items(A, count, cur, nxt, nxt_cnt, bit_cur, bit_nxt, lane, my_defer, [&](const unsigned item) {
         const unsigned shot = item >> kShotShift, b = item & kBlkMask;
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
         if (lane == 0) atomicAnd(&fl[b], ~bit_cur);
-        if (A.keys) {
-            const float tau = __uint_as_float(A.kmin[(A.round % 3) * A.nshot + shot]) + A.band;
-            if (A.keys[(long long)shot * A.nblk + b] > tau) {  // not its time yet
-                schedule(A, shot, fl, lane == 0 ? b : 0xFFFFFFFFu, bit_nxt, nxt, nxt_cnt, lane);
-                my_defer += 1;
-                continue;
-            }
-        }
         const int bx = b % A.nbx, rest = b / A.nbx, by = rest % A.nby, bz = rest / A.nby;
         const long long base = (long long)b * 512;
         const bool vxy = (bx * 8 + tx < A.n4x) && (by * 8 + ty < A.n4y);
@@ -387,7 +425,7 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
         my_visits += 1;
         my_iters += it;
         __syncthreads();
-    }
+    });
     if (lane == 0 && A.stats) {
         atomicAdd(&A.stats[0], my_visits);
         atomicAdd(&A.stats[1], my_iters);
@@ -515,7 +553,7 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
     const long long rest = b / nbx;
     const int by = (int)(rest % nby), bz = (int)(rest / nby);
     const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
-    float kmin = INFINITY;
+    float kmin = kInf;
     for (int t = lane; t < B3; t += 64) {
         const int tx = t % B, ty = (t / B) % B, tz = t / (B * B);
         const int x = bx * B + tx, y = by * B + ty, z = bz * B + tz;
@@ -524,14 +562,18 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
             const float px = x * dx, py = y * dy, pz = z * dz;
             const float ex = px - sx, ey = py - sy, ez = pz - sz;
             const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-            const float p = pred(s, x, y, z, px, py, pz, dist);
+            float p = pred(s, x, y, z, px, py, pz, dist);
+            // a non-finite prediction (exponent all ones) counts as unreached;
+            // tested on the bits, which survive no-NaN code generation
+            if ((__float_as_uint(p) & 0x7F800000u) == 0x7F800000u) p = kInf;
             kmin = fminf(kmin, p);
             if (dist >= r_cold) v = fminf(scale * p, kInf);
         }
         if (write_values) T[s * vol + b * B3 + t] = v;
     }
     kmin = wave_min(kmin);
-    if (lane == 0) keys[s * nblk + b] = isfinite(kmin) ? fmaxf(kmin, 0.0f) : 0.0f;
+    // no usable prediction in the block: schedule it as early as possible
+    if (lane == 0) keys[s * nblk + b] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
 }
 
 // Judge: largest residual G(T) - T over the domain cells of each shot
