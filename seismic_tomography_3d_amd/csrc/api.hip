@@ -108,10 +108,10 @@ int st_ctx_create(int device, st_ctx **out) {
         ST_REQUIRE(device >= 0 && device < n, ST_ERR_ARG, "no such HIP device");
         set_device(ctx);
         ST_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        ST_HIP(hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault));
+        ST_HIP(hipHostMalloc(&ctx->host_pinned, 16384, hipHostMallocDefault));
         for (auto &e : ctx->ev) ST_HIP(hipEventCreate(&e));
-        ctx->counters.reserve(256);
-        ST_HIP(hipMemset(ctx->counters.ptr, 0, 256));
+        ctx->counters.reserve(8192);
+        ST_HIP(hipMemset(ctx->counters.ptr, 0, 8192));
     });
     if (rc != ST_OK) {
         st_ctx_destroy(ctx);
@@ -130,7 +130,7 @@ int st_ctx_destroy(st_ctx *ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (auto *b : {&ctx->slow_zxy, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
-                    &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->src, &ctx->rcv,
+                    &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->wstats, &ctx->src, &ctx->rcv,
                     &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred})
         b->release();
     for (auto &e : ctx->ev)
@@ -213,7 +213,8 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
     int batch = std::max(1, std::min({o.batch, nshot, 1023}));
     size_t free_b = 0, total_b = 0;
     ST_HIP(hipMemGetInfo(&free_b, &total_b));
-    const size_t per_shot = sizeof(float) * g.vol + 3 * sizeof(unsigned) * g.nblk +
+    const size_t per_shot = sizeof(float) * g.vol + sizeof(unsigned) * g.nblk +
+                            2 * fim_list_bytes(g.nblk, 1) +
                             (warm ? sizeof(float) * g.nblk : 0) +
                             (T_pred ? sizeof(float) * npts : 0) + (T_out ? sizeof(float) * npts : 0);
     const size_t have = free_b + ctx->T.bytes + ctx->flags.bytes + ctx->list[0].bytes +
@@ -222,8 +223,8 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
 
     ctx->T.reserve(sizeof(float) * g.vol * batch);
     ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
-    ctx->list[0].reserve(sizeof(unsigned) * g.nblk * batch);
-    ctx->list[1].reserve(sizeof(unsigned) * g.nblk * batch);
+    ctx->list[0].reserve(fim_list_bytes(g.nblk, batch));
+    ctx->list[1].reserve(fim_list_bytes(g.nblk, batch));
     ctx->src.reserve(sizeof(float) * 3 * (nshot + batch));
     ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
     ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
@@ -431,8 +432,8 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
         int batch = std::max(1, std::min({o.batch, nshot, 1023}));
         ctx->T.reserve(sizeof(float) * g.vol * batch);
         ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
-        ctx->list[0].reserve(sizeof(unsigned) * g.nblk * batch);
-        ctx->list[1].reserve(sizeof(unsigned) * g.nblk * batch);
+        ctx->list[0].reserve(fim_list_bytes(g.nblk, batch));
+        ctx->list[1].reserve(fim_list_bytes(g.nblk, batch));
         ctx->src.reserve(sizeof(float) * 3 * nshot);
         ctx->rcv.reserve(sizeof(float) * 3 * nrcv);
         ctx->picks.reserve(sizeof(float) * (long long)nshot * nrcv);

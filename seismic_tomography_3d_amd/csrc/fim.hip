@@ -30,13 +30,22 @@
 
 namespace st {
 
+constexpr int kSeg = kListSegments;
+constexpr int kCntStride = 32;                       // one 128-byte line per counter
+constexpr int kRoundsIdx = 3 * kSeg * kCntStride;    // rounds with a non-empty list
+constexpr int kStatsIdx = kRoundsIdx + kCntStride;   // 3 x u64 summed statistics
+constexpr int kCntWords = kStatsIdx + 8;
+
 struct FimArgs {
     const float *S;            // slowness, block layout (shared by shots)
     float *T;                  // travel times, shot-major block layout
     unsigned *flags;           // per (shot, block): bit p = scheduled for parity p
-    unsigned *list0, *list1;   // active lists by round parity
-    unsigned *cnt;             // [0..2] list lengths (rotating), [3] rounds run
-    unsigned long long *stats; // [0] visits, [1] inner iterations
+    unsigned *list0, *list1;   // active lists by round parity, kSeg segments of cap
+    long long cap;             // items per segment (nblk * nshot)
+    unsigned *cnt;             // segment lengths: cnt[(slot * kSeg + seg) * kCntStride]
+                               // for rotating slots 0..2; rounds run at cnt[kRoundsIdx]
+    unsigned long long *stats; // per wave gw: [3 gw] visits, [3 gw + 1] inner
+                               // iterations, [3 gw + 2] deferrals
     int nbx, nby, nbz;
     int n4x, n4y, n4z;
     long long vol, nblk;
@@ -113,22 +122,58 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
     }
 }
 
-// Walks this workgroup's items of the current list (i = blockIdx.x + j *
-// gridDim.x, the same interleaving for every round) 64 at a time: lane l
+// Round kernels run kWaves independent waves per workgroup (each with its
+// own LDS tile and item stream): workgroup slots per CU, not registers, cap
+// single-wave launches at four waves per SIMD.  Waves exchange nothing, so
+// they order their own LDS traffic with wave-scope fences instead of
+// s_barrier (which would couple waves with different trip counts).
+constexpr int kWaves = 2;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Segment lengths of this round's list: lane k < kSeg gets the inclusive
+// prefix over segments 0..k in incl, every lane the total in count.
+__device__ __forceinline__ void round_counts(const FimArgs &A, int lane, unsigned &incl,
+                                             unsigned &count) {
+    const int slot = A.round % 3;
+    unsigned v = lane < kSeg ? A.cnt[(slot * kSeg + lane) * kCntStride] : 0u;
+#pragma unroll
+    for (int o = 1; o < kSeg; o <<= 1) {
+        const unsigned t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    incl = v;
+    count = (unsigned)__builtin_amdgcn_readlane((int)v, kSeg - 1);
+}
+
+// Walks this wave's items of the current list (i = gw + j * G for global
+// wave gw of G, the same interleaving for every round) 64 at a time: lane l
 // reads item j0 + l.  With arrival ordering, the items whose block is not
 // due yet (predicted arrival above its shot's earliest scheduled key plus
 // the band) move to the next list right here, all lanes at once with one
 // append; the others are visited one after another by the whole wave.
 template <class Visit>
-__device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, const unsigned *cur,
-                                          unsigned *nxt, unsigned *nxt_cnt, unsigned bit_cur,
-                                          unsigned bit_nxt, int lane,
-                                          unsigned long long &my_defer, Visit visit) {
-    const unsigned long long G = gridDim.x;
-    for (unsigned long long j0 = 0; blockIdx.x + j0 * G < count; j0 += 64) {
-        const unsigned long long i = blockIdx.x + (j0 + lane) * G;
+__device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, unsigned incl,
+                                          const unsigned *cur, unsigned *nxt, unsigned *nxt_cnt,
+                                          unsigned bit_cur, unsigned bit_nxt, int lane,
+                                          unsigned gw, unsigned G, unsigned long long &my_defer,
+                                          Visit visit) {
+    for (unsigned long long j0 = 0; gw + j0 * G < count; j0 += 64) {
+        const unsigned long long i = gw + (j0 + lane) * G;
         const bool in = i < count;
-        const unsigned item = in ? cur[i] : 0u;
+        // flat index -> (segment, offset): incl holds, in lane k < kSeg, the
+        // inclusive prefix of the segment lengths
+        unsigned seg = 0, before = 0;
+#pragma unroll
+        for (int k = 0; k < kSeg - 1; k++) {
+            const unsigned ik = (unsigned)__builtin_amdgcn_readlane((int)incl, k);
+            if (i >= ik) { seg = k + 1; before = ik; }
+        }
+        const unsigned item = in ? cur[seg * A.cap + (i - before)] : 0u;
         bool run = in;
         if (A.keys) {
             bool fresh = false;
@@ -170,23 +215,26 @@ __device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, cons
 // ---------------------------------------------------------------------------
 // 4^3 blocks: lane = tx + 4 ty + 16 tz, one cell per lane, Jacobi sweeps.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
-    __shared__ float tile[216];  // (4+2)^3, index x + 6 y + 36 z
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * kWaves) void fim_round4(FimArgs A) {
+    __shared__ float tiles[kWaves][216];  // (4+2)^3, index x + 6 y + 36 z
+    const int lane = threadIdx.x & 63;
+    float *tile = tiles[threadIdx.x >> 6];
+    const unsigned gw = blockIdx.x * kWaves + (threadIdx.x >> 6), G = gridDim.x * kWaves;
     const int p = A.round & 1;
-    const unsigned count = A.cnt[A.round % 3];
-    if (blockIdx.x == 0) {
-        if (lane == 0) {
-            A.cnt[(A.round + 2) % 3] = 0;
-            if (count) atomicAdd(&A.cnt[3], 1u);
-        }
+    unsigned incl = 0, count = 0;
+    round_counts(A, lane, incl, count);
+    if (gw == 0) {
+        if (lane < kSeg) A.cnt[(((A.round + 2) % 3) * kSeg + lane) * kCntStride] = 0;
+        if (lane == 0 && count) atomicAdd(&A.cnt[kRoundsIdx], 1u);
         if (A.keys)
             for (int j = lane; j < A.nshot; j += 64) A.kmin[((A.round + 2) % 3) * A.nshot + j] = kInfBits;
     }
     if (count == 0) return;
     const unsigned *cur = p ? A.list1 : A.list0;
-    unsigned *nxt = p ? A.list0 : A.list1;
-    unsigned *nxt_cnt = &A.cnt[(A.round + 1) % 3];
+    // this wave appends to segment gw mod kSeg of the next list
+    const int myseg = (int)(gw & (kSeg - 1));
+    unsigned *nxt = (p ? A.list0 : A.list1) + myseg * A.cap;
+    unsigned *nxt_cnt = &A.cnt[(((A.round + 1) % 3) * kSeg + myseg) * kCntStride];
     const unsigned bit_cur = 1u << p, bit_nxt = 1u << (p ^ 1);
 
     const int tx = lane & 3, ty = (lane >> 2) & 3, tz = lane >> 4;
@@ -194,7 +242,7 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
     const int nbxy = A.nbx * A.nby;
     unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
-    for_items(A, count, cur, nxt, nxt_cnt, bit_cur, bit_nxt, lane, my_defer, [&](const unsigned item) {
+    for_items(A, count, incl, cur, nxt, nxt_cnt, bit_cur, bit_nxt, lane, gw, G, my_defer, [&](const unsigned item) {
         const unsigned shot = item >> kShotShift, b = item & kBlkMask;
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
@@ -226,7 +274,7 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
             }
             tile[ti] = hv;
         }
-        __syncthreads();
+        wave_sync();
 
         const float hs = A.h * s, hs2 = hs * hs;
         const float v0 = v;
@@ -239,12 +287,12 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
             const float t = godunov(a, bb, c, hs, hs2);
             const float nv = valid ? fminf(v, t) : v;
             const bool ch = (v - nv) > A.eps;
-            __syncthreads();
+            wave_sync();
             tile[tc] = nv;
             v = nv;
             ++it;
             more = __ballot(ch) != 0ull;
-            __syncthreads();
+            wave_sync();
             if (!more) break;
         }
 
@@ -275,12 +323,14 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
         schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane);
         my_visits += 1;
         my_iters += it;
-        __syncthreads();
+        wave_sync();
     });
-    if (lane == 0 && A.stats) {
-        atomicAdd(&A.stats[0], my_visits);
-        atomicAdd(&A.stats[1], my_iters);
-        if (my_defer) atomicAdd(&A.stats[2], my_defer);
+    if (lane == 0 && A.stats && (my_visits | my_defer)) {
+        // each wave owns its slot: no atomics on shared counters
+        unsigned long long *w = A.stats + 3ull * gw;
+        w[0] += my_visits;
+        w[1] += my_iters;
+        w[2] += my_defer;
     }
 }
 
@@ -289,23 +339,26 @@ __global__ __launch_bounds__(64) void fim_round4(FimArgs A) {
 // x/y neighbours come through the LDS tile, z neighbours from registers
 // (Gauss-Seidel along z, alternating direction each sweep).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
-    __shared__ float tile[1000];  // (8+2)^3, index x + 10 y + 100 z
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(64 * kWaves) void fim_round8(FimArgs A) {
+    __shared__ float tiles[kWaves][1000];  // (8+2)^3, index x + 10 y + 100 z
+    const int lane = threadIdx.x & 63;
+    float *tile = tiles[threadIdx.x >> 6];
+    const unsigned gw = blockIdx.x * kWaves + (threadIdx.x >> 6), G = gridDim.x * kWaves;
     const int p = A.round & 1;
-    const unsigned count = A.cnt[A.round % 3];
-    if (blockIdx.x == 0) {
-        if (lane == 0) {
-            A.cnt[(A.round + 2) % 3] = 0;
-            if (count) atomicAdd(&A.cnt[3], 1u);
-        }
+    unsigned incl = 0, count = 0;
+    round_counts(A, lane, incl, count);
+    if (gw == 0) {
+        if (lane < kSeg) A.cnt[(((A.round + 2) % 3) * kSeg + lane) * kCntStride] = 0;
+        if (lane == 0 && count) atomicAdd(&A.cnt[kRoundsIdx], 1u);
         if (A.keys)
             for (int j = lane; j < A.nshot; j += 64) A.kmin[((A.round + 2) % 3) * A.nshot + j] = kInfBits;
     }
     if (count == 0) return;
     const unsigned *cur = p ? A.list1 : A.list0;
-    unsigned *nxt = p ? A.list0 : A.list1;
-    unsigned *nxt_cnt = &A.cnt[(A.round + 1) % 3];
+    // this wave appends to segment gw mod kSeg of the next list
+    const int myseg = (int)(gw & (kSeg - 1));
+    unsigned *nxt = (p ? A.list0 : A.list1) + myseg * A.cap;
+    unsigned *nxt_cnt = &A.cnt[(((A.round + 1) % 3) * kSeg + myseg) * kCntStride];
     const unsigned bit_cur = 1u << p, bit_nxt = 1u << (p ^ 1);
 
     const int tx = lane & 7, ty = lane >> 3;
@@ -313,7 +366,7 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
     const int nbxy = A.nbx * A.nby;
     unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
-    for_items(A, count, cur, nxt, nxt_cnt, bit_cur, bit_nxt, lane, my_defer, [&](const unsigned item) {
+    for_items(A, count, incl, cur, nxt, nxt_cnt, bit_cur, bit_nxt, lane, gw, G, my_defer, [&](const unsigned item) {
         const unsigned shot = item >> kShotShift, b = item & kBlkMask;
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
@@ -350,7 +403,7 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
             }
             tile[ti] = hv;
         }
-        __syncthreads();
+        wave_sync();
 
         int it = 0;
         bool more = true;
@@ -368,9 +421,9 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
                 const float nv = valid ? fminf(vz[k + 1], t) : vz[k + 1];
                 ch |= (vz[k + 1] - nv) > A.eps;
                 vz[k + 1] = nv;
-                __syncthreads();
+                wave_sync();
                 tile[ti] = nv;
-                __syncthreads();
+                wave_sync();
             };
             if ((it & 1) == 0) {
                 #pragma unroll
@@ -424,12 +477,14 @@ __global__ __launch_bounds__(64) void fim_round8(FimArgs A) {
         schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane);
         my_visits += 1;
         my_iters += it;
-        __syncthreads();
+        wave_sync();
     });
-    if (lane == 0 && A.stats) {
-        atomicAdd(&A.stats[0], my_visits);
-        atomicAdd(&A.stats[1], my_iters);
-        if (my_defer) atomicAdd(&A.stats[2], my_defer);
+    if (lane == 0 && A.stats && (my_visits | my_defer)) {
+        // each wave owns its slot: no atomics on shared counters
+        unsigned long long *w = A.stats + 3ull * gw;
+        w[0] += my_visits;
+        w[1] += my_iters;
+        w[2] += my_defer;
     }
 }
 
@@ -466,7 +521,11 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
                              float dx, float dy, float dz, int B, int nbx, int nby,
                              long long vol, long long nblk, const float *keys, unsigned *kmin) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) { cnt[0] = nshot; cnt[1] = 0; cnt[2] = 0; cnt[3] = 0; }
+    if (j == 0) {  // the sources form segment 0 of the first list
+        for (int k = 0; k < 3 * kSeg; k++) cnt[k * kCntStride] = 0;
+        cnt[0] = nshot;
+        cnt[kRoundsIdx] = 0;
+    }
     if (j >= nshot) return;
     const float sx = src[3 * j], sy = src[3 * j + 1], sz = src[3 * j + 2];
     const int ix = (int)(sx / dx), iy = (int)(sy / dy), iz = (int)(sz / dz);
@@ -619,6 +678,28 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
 }
 }  // namespace st
 
+// Sums the per-wave statistics slots into out[0..2] (one workgroup).
+__global__ __launch_bounds__(256) void sum_wave_stats(const unsigned long long *__restrict__ w,
+                                                      int nw, unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long part[3][256];
+    unsigned long long a = 0, b = 0, c = 0;
+    for (int i = threadIdx.x; i < nw; i += 256) {
+        a += w[3 * i];
+        b += w[3 * i + 1];
+        c += w[3 * i + 2];
+    }
+    part[0][threadIdx.x] = a;
+    part[1][threadIdx.x] = b;
+    part[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 3; k++) part[k][threadIdx.x] += part[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = part[threadIdx.x][0];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -648,7 +729,9 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     const int B = o.block;
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     unsigned *cnt = ctx->counters.as<unsigned>();
-    unsigned long long *dstats = reinterpret_cast<unsigned long long *>(cnt + 8);
+    unsigned long long *dstats = reinterpret_cast<unsigned long long *>(cnt + kStatsIdx);
+    ctx->list[0].reserve(fim_list_bytes(g.nblk, nshot));
+    ctx->list[1].reserve(fim_list_bytes(g.nblk, nshot));
     const int tpb = 256;
 
     if (keys) ctx->kmin.reserve(sizeof(unsigned) * 3 * std::max(nshot, 1));
@@ -668,10 +751,11 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     A.S = block_model(ctx, B);
     A.T = ctx->T.as<float>();
     A.flags = ctx->flags.as<unsigned>();
+    A.cap = g.nblk * (long long)nshot;
     A.list0 = ctx->list[0].as<unsigned>();
     A.list1 = ctx->list[1].as<unsigned>();
     A.cnt = cnt;
-    A.stats = dstats;
+    A.stats = nullptr;
     A.nbx = g.nbx; A.nby = g.nby; A.nbz = g.nbz;
     A.n4x = g.n4x; A.n4y = g.n4y; A.n4z = g.n4z;
     A.vol = g.vol; A.nblk = g.nblk;
@@ -683,10 +767,27 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     A.nshot = nshot;
     A.band = o.band;
 
+    // o.grid counts waves; a launch has o.grid / kWaves workgroups
     int grid = o.grid;
     if (grid <= 0) grid = 256 * 16;
+    grid = std::max(1, (grid + kWaves - 1) / kWaves);
+    const int nwaves = grid * kWaves;
+    if (stats) {
+        ctx->wstats.reserve(sizeof(unsigned long long) * 3 * nwaves);
+        ST_HIP(hipMemsetAsync(ctx->wstats.ptr, 0, sizeof(unsigned long long) * 3 * nwaves,
+                              ctx->stream));
+        A.stats = ctx->wstats.as<unsigned long long>();
+    }
     const int K = o.rounds_per_sync > 0 ? o.rounds_per_sync : 6;
     unsigned *hcnt = static_cast<unsigned *>(ctx->host_pinned);
+    // two snapshots of a slot's segment counters (polled one chunk behind)
+    unsigned *hseg[2] = {static_cast<unsigned *>(ctx->host_pinned) + 2048,
+                         static_cast<unsigned *>(ctx->host_pinned) + 2048 + kSeg * kCntStride};
+    auto seg_total = [&](const unsigned *h) {
+        unsigned long long t = 0;
+        for (int k = 0; k < kSeg; k++) t += h[k * kCntStride];
+        return t;
+    };
     const long long max_rounds = 4LL * (g.nbx + g.nby + g.nbz) * 64 + 4096;
 
     ST_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -703,6 +804,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         mix((long long)(uintptr_t)A.T); mix((long long)(uintptr_t)A.S); mix((long long)(uintptr_t)A.keys);
         mix((long long)(uintptr_t)A.flags); mix((long long)(uintptr_t)A.list0);
         mix((long long)(uintptr_t)A.list1); mix((long long)(uintptr_t)A.kmin);
+        mix((long long)(uintptr_t)A.stats);
         mix(nshot); mix(B); mix(grid); mix(A.nblk); mix(A.max_inner);
         mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
         mix((long long)__builtin_bit_cast(unsigned, A.h));
@@ -715,9 +817,9 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
             for (int k = 0; k < 6; k++) {
                 A.round = k;
                 if (B == 8)
-                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64), 0, ctx->stream, A);
+                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
                 else
-                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64), 0, ctx->stream, A);
+                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
             }
             ST_HIP(hipStreamEndCapture(ctx->stream, &ctx->graph));
             ST_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
@@ -733,18 +835,19 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
             for (int k = 0; k < Kg; k++, r++) {
                 A.round = (int)(r % 6);
                 if (B == 8)
-                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64), 0, ctx->stream, A);
+                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
                 else
-                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64), 0, ctx->stream, A);
+                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
             }
         }
         ST_HIP(hipGetLastError());
-        ST_HIP(hipMemcpyAsync(&hcnt[slot], &cnt[r % 3], sizeof(unsigned), hipMemcpyDeviceToHost,
+        ST_HIP(hipMemcpyAsync(hseg[slot], &cnt[(r % 3) * kSeg * kCntStride],
+                              sizeof(unsigned) * kSeg * kCntStride, hipMemcpyDeviceToHost,
                               ctx->stream));
         ST_HIP(hipEventRecord(ctx->ev[slot], ctx->stream));
         if (pending >= 0) {
             ST_HIP(hipEventSynchronize(ctx->ev[pending]));
-            if (hcnt[pending] == 0) break;
+            if (seg_total(hseg[pending]) == 0) break;
         }
         pending = slot;
         slot ^= 1;
@@ -752,7 +855,11 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     }
     ST_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     if (stats) {
-        ST_HIP(hipMemcpyAsync(hcnt + 4, cnt + 3, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+        hipLaunchKernelGGL(sum_wave_stats, dim3(1), dim3(256), 0, ctx->stream,
+                           ctx->wstats.as<unsigned long long>(), nwaves, dstats);
+        ST_HIP(hipGetLastError());
+        ST_HIP(hipMemcpyAsync(hcnt + 4, cnt + kRoundsIdx, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              ctx->stream));
         ST_HIP(hipMemcpyAsync(hcnt + 8, dstats, 3 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
@@ -765,7 +872,6 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         float ms = 0.f;
         ST_HIP(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         stats->solve_ms += ms;
-        ST_HIP(hipMemsetAsync(dstats, 0, 3 * sizeof(unsigned long long), ctx->stream));
     }
 }
 

@@ -55,6 +55,16 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Active lists of the block FIM are split into kListSegments append
+// segments, each with its own counter on its own 128-byte line, so that the
+// per-visit append atomics of a round spread over as many addresses.  A
+// segment can receive every block of the batch, so a list holds
+// kListSegments * nblk * nshot items.
+constexpr int kListSegments = 16;
+inline size_t fim_list_bytes(long long nblk, long long nshot) {
+    return sizeof(unsigned) * (size_t)kListSegments * (size_t)nblk * (size_t)std::max(nshot, 1LL);
+}
+
 // Block-major volume geometry.  The FIM domain is the reference's
 // pad-to-4 extent (n4*); blocks of edge B tile the pad-to-B extent, cells
 // beyond n4* are outside the domain and stay at kInf.
@@ -111,6 +121,7 @@ struct st_ctx {
     st::DevBuf flags;                  // batch x nblk schedule bits
     st::DevBuf list[2];                // active lists (by round parity)
     st::DevBuf counters;               // list counts, stats
+    st::DevBuf wstats;                 // per-wave FIM statistics slots
     st::DevBuf src, rcv, picks, scratch;
     st::DevBuf keys, kmin, judge, pred;   // warm-start state
     void *host_pinned = nullptr;       // small pinned staging area

@@ -620,8 +620,8 @@ int st_surrogate_make_data(st_ctx *ctx, int n_models, int spm, float zmax_src, u
         const BlockGeo bg = make_geo(ctx->nx, ctx->ny, ctx->nz, o.block);
         ctx->T.reserve(sizeof(float) * bg.vol * spm);
         ctx->flags.reserve(sizeof(unsigned) * bg.nblk * spm);
-        ctx->list[0].reserve(sizeof(unsigned) * bg.nblk * spm);
-        ctx->list[1].reserve(sizeof(unsigned) * bg.nblk * spm);
+        ctx->list[0].reserve(fim_list_bytes(bg.nblk, spm));
+        ctx->list[1].reserve(fim_list_bytes(bg.nblk, spm));
         ctx->src.reserve(sizeof(float) * 3 * spm);
         const CubeGeo cg = cube_geo(ctx, n.nc);
         std::mt19937_64 rng(seed);

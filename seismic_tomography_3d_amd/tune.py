@@ -1,8 +1,8 @@
 """Autotuner for the batched FIM launch shape.
 
-Chooses the block edge (4 or 8), the number of workgroups per round launch
-(which sets resident waves per CU: one 64-lane workgroup per visit slot) and
-the inner-iteration cap by timing the device solve (HIP events inside the
+Chooses the block edge (4 or 8), the number of waves per round launch
+(resident waves per CU; each wave walks its own share of the active list)
+and the inner-iteration cap by timing the device solve (HIP events inside the
 library) on a sample of the caller's shots.  Results are cached per model
 shape and shot count for the life of the process.
 """
@@ -33,7 +33,7 @@ def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=64, log=None
     if candidates is None:
         candidates = []
         for B in ([block] if block else [4, 8]):
-            for wpc in ([grid // CUS] if grid else ([8] if B == 4 else [8, 16, 32])):
+            for wpc in ([grid // CUS] if grid else ([16] if B == 4 else [16, 32])):
                 for mi in ([max_inner] if max_inner else ([16] if B == 4 else [16, 24])):
                     candidates.append(dict(block=B, grid=wpc * CUS, max_inner=mi))
     best, best_ms = None, float("inf")
