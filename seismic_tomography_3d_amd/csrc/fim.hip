@@ -635,6 +635,43 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
     if (lane == 0) keys[s * nblk + b] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
 }
 
+// Keys only (ordering-only warm starts): one thread per (shot, block), the
+// block's predicted first arrival taken as the smallest prediction on the
+// 3 x 3 x 3 lattice of its corners, edge and face midpoints and centre
+// (clipped to the model).  An ordering key needs no more: the schedule stays
+// exact whatever the keys are.
+template <class Pred>
+__global__ void warm_keys(float *__restrict__ keys, const float *__restrict__ src, int nshot, int B,
+                          int nbx, int nby, long long nblk, int nx, int ny, int nz, float dx,
+                          float dy, float dz, Pred pred) {
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (long long)nshot * nblk) return;
+    const int s = (int)(w / nblk);
+    const long long b = w - (long long)s * nblk;
+    const int bx = (int)(b % nbx);
+    const long long rest = b / nbx;
+    const int by = (int)(rest % nby), bz = (int)(rest / nby);
+    const int x0 = bx * B, y0 = by * B, z0 = bz * B;
+    const int x1 = min(x0 + B - 1, nx - 1), y1 = min(y0 + B - 1, ny - 1), z1 = min(z0 + B - 1, nz - 1);
+    const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+    float kmin = kInf;
+    for (int k = 0; k < 3; k++) {
+        const int z = k == 0 ? z0 : (k == 1 ? (z0 + z1) / 2 : z1);
+        for (int j = 0; j < 3; j++) {
+            const int y = j == 0 ? y0 : (j == 1 ? (y0 + y1) / 2 : y1);
+            for (int i = 0; i < 3; i++) {
+                const int x = i == 0 ? x0 : (i == 1 ? (x0 + x1) / 2 : x1);
+                const float px = x * dx, py = y * dy, pz = z * dz;
+                const float ex = px - sx, ey = py - sy, ez = pz - sz;
+                float p = pred(s, x, y, z, px, py, pz, sqrtf(ex * ex + ey * ey + ez * ez));
+                if ((__float_as_uint(p) & 0x7F800000u) == 0x7F800000u) p = kInf;
+                kmin = fminf(kmin, p);
+            }
+        }
+    }
+    keys[w] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
+}
+
 // Judge: largest residual G(T) - T over the domain cells of each shot
 // (source node excluded).  A converged solve from an upper bound has none:
 // every cell holds the update of its final neighbours.  A positive residual
@@ -880,10 +917,16 @@ void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nsho
                     float delta, float r_cold, bool values, float *dkeys) {
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     FinePred fp{dpred, ctx->nx, ctx->nz, (long long)ctx->nx * ctx->ny * ctx->nz};
-    if (!values) {
+    if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,
                            ctx->T.as<float4>(), n4);
+        const long long nk = (long long)nshot * g.nblk;
+        hipLaunchKernelGGL(warm_keys<FinePred>, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0,
+                           ctx->stream, dkeys, dsrc, nshot, B, g.nbx, g.nby, g.nblk, ctx->nx,
+                           ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, fp);
+        ST_HIP(hipGetLastError());
+        return;
     }
     hipLaunchKernelGGL(warm_fill<FinePred>, dim3((unsigned)(g.nblk * nshot)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
@@ -900,10 +943,16 @@ void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dm
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     CoarsePred cp{dQ, nc, (nc - 1) / ((ctx->nx - 1) * ctx->dx), (nc - 1) / ((ctx->ny - 1) * ctx->dy),
                   (nc - 1) / ((ctx->nz - 1) * ctx->dz), 1.0f / vref, dmin};
-    if (!values) {
+    if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,
                            ctx->T.as<float4>(), n4);
+        const long long nk = (long long)nshot * g.nblk;
+        hipLaunchKernelGGL(warm_keys<CoarsePred>, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0,
+                           ctx->stream, dkeys, dsrc, nshot, B, g.nbx, g.nby, g.nblk, ctx->nx,
+                           ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, cp);
+        ST_HIP(hipGetLastError());
+        return;
     }
     hipLaunchKernelGGL(warm_fill<CoarsePred>, dim3((unsigned)(g.nblk * nshot)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,

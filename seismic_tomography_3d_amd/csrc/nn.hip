@@ -100,95 +100,157 @@ void gemm(hipStream_t st, const GemmArgs &g) {
 // ---------------------------------------------------------------------------
 // Implicit-GEMM 3D convolution, forward (stride 1, zero "same" padding):
 //   Y[s, v, n] = act(b[n] + sum_{tap, c} X[s, v + off(tap), c] * W[tap*cin + c, n])
-// One workgroup (4 waves) per 4x4x4 output brick of one sample.  The
-// (4 + KS - 1)^3 input halo, all channels, is staged once in LDS as BF16
-// (row stride cinp + 4 halves to spread LDS banks); wave w owns the 16
-// voxels of brick slab z = w and all output channels, so each A fragment
-// read from LDS feeds NT v_mfma_f32_16x16x16_bf16.  K = tap*cinp + c runs
-// in steps of 16: a lane's four consecutive k share one tap (cinp % 4 == 0),
+// where X is the channel concatenation of X0 (C0 channels; at half
+// resolution, nearest-upsampled, when up0) and X1 (C1 channels, may be
+// empty) -- the decoder's upsample + skip concat happens while staging.
+//
+// One workgroup (4 waves) per BX x 4 x 4 output brick of one sample.  The
+// (BX + KS - 1) x (4 + KS - 1)^2 input halo, all channels, is staged once in
+// LDS as BF16 (row stride CINP + 4 halves to spread banks).  M-tile t holds
+// the 16 voxels x = 4 (t >> 2) + (m & 3), y = m >> 2, z = t & 3; wave w owns
+// m-tiles w and w + 4 (BX = 8) so that every B fragment feeds MT MFMAs, and
+// every A fragment NT.  K = tap * CINP + c in steps of 16
+// (v_mfma_f32_16x16x16_bf16): a lane's four consecutive k share one tap,
 // so its A fragment is one 8-byte LDS read and its B fragment one 8-byte
-// load from the packed weights Wp[tap][n][c] (n padded to 16*NT, c to
-// cinp, an extra all-zero tap absorbing the K padding).
+// load of the packed weights Wp[tap][n][c] (n padded to 16 NT, c to CINP,
+// a zero tap absorbing the K padding of CINP = 8).  The B fragments of the
+// next tap group are loaded while the current one computes.
 // ---------------------------------------------------------------------------
-template <int KS, int NT>
-__global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X,
+template <int KS, int CINP, int NT, int BX>
+__global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, int C0, int up0,
+                                                  const float *__restrict__ X1, int C1,
                                                   const unsigned short *__restrict__ Wp,
                                                   const float *__restrict__ bias,
-                                                  float *__restrict__ Y, int e, int cin, int cinp,
-                                                  int cout, int relu) {
-    constexpr int H = 4 + KS - 1, H3 = H * H * H, TAPS = KS * KS * KS, HF = KS / 2;
-    extern __shared__ unsigned short halo[];
-    const int cst = cinp + 4;
+                                                  float *__restrict__ Y, int e, int cout, int relu) {
+    constexpr int HF = KS / 2, HX = BX + KS - 1, H = 4 + KS - 1, NH = HX * H * H;
+    constexpr int TAPS = KS * KS * KS, CST = CINP + 4, MT = BX / 4, COUTP = 16 * NT;
+    // tap groups of 16-wide k steps: CINP >= 16 -> one tap of CINP / 16
+    // steps; CINP = 8 -> two taps in one step
+    constexpr int GS = CINP >= 16 ? CINP / 16 : 1;
+    constexpr int NG = CINP >= 16 ? TAPS : (TAPS + 1) / 2;
+    static_assert(CINP == 8 || CINP % 16 == 0, "channel padding");
+    __shared__ unsigned short halo[NH * CST];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nb = (e + 3) >> 2;
-    const int bid = blockIdx.x;
-    const int bx = bid % nb, by = (bid / nb) % nb, bz = (bid / (nb * nb)) % nb;
-    const int s = bid / (nb * nb * nb);
+    const int nbx = (e + BX - 1) / BX, nb4 = (e + 3) >> 2;
+    int bid = blockIdx.x;
+    const int bx = bid % nbx; bid /= nbx;
+    const int by = bid % nb4; bid /= nb4;
+    const int bz = bid % nb4;
+    const int s = bid / nb4;
     const long long V = (long long)e * e * e;
-    const float *Xs = X + (long long)s * V * cin;
-    const int ox = 4 * bx - HF, oy = 4 * by - HF, oz = 4 * bz - HF;
+    const int eh = e >> 1;
+    const long long Vh = (long long)eh * eh * eh;
+    const int ox = BX * bx - HF, oy = 4 * by - HF, oz = 4 * bz - HF;
+    const int cin = C0 + C1;
 
-    // stage the halo: 4 channels per item
-    const int c4 = cinp >> 2;
-    for (int i = tid; i < H3 * c4; i += 256) {
-        const int hv = i / c4, c = (i - hv * c4) * 4;
-        const int hx = hv % H, hy = (hv / H) % H, hz = hv / (H * H);
+    // stage the halo, 4 channels per item
+    constexpr int C4 = CINP / 4;
+    for (int i = tid; i < NH * C4; i += 256) {
+        const int hv = i / C4, c = (i - hv * C4) * 4;
+        const int hx = hv % HX, hy = (hv / HX) % H, hz = hv / (HX * H);
         const int xx = ox + hx, yy = oy + hy, zz = oz + hz;
-        float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+        float f[4] = {0.f, 0.f, 0.f, 0.f};
         if (c < cin && xx >= 0 && yy >= 0 && zz >= 0 && xx < e && yy < e && zz < e) {
-            const float *p = Xs + (xx + (long long)e * (yy + (long long)e * zz)) * cin + c;
-            if ((cin & 3) == 0) {
-                f = *reinterpret_cast<const float4 *>(p);
+            const float *p;
+            int cc, nav;
+            if (c < C0) {
+                p = up0 ? X0 + ((long long)s * Vh + (xx >> 1) + (long long)eh * ((yy >> 1) + (long long)eh * (zz >> 1))) * C0
+                        : X0 + ((long long)s * V + xx + (long long)e * (yy + (long long)e * zz)) * C0;
+                cc = c;
+                nav = C0 - c;
             } else {
-                if (c + 0 < cin) f.x = p[0];
-                if (c + 1 < cin) f.y = p[1];
-                if (c + 2 < cin) f.z = p[2];
-                if (c + 3 < cin) f.w = p[3];
+                p = X1 + ((long long)s * V + xx + (long long)e * (yy + (long long)e * zz)) * C1;
+                cc = c - C0;
+                nav = C1 - cc;
+            }
+            if (nav >= 4 && ((C0 | C1) & 3) == 0) {
+                const float4 v = *reinterpret_cast<const float4 *>(p + cc);
+                f[0] = v.x; f[1] = v.y; f[2] = v.z; f[3] = v.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k < nav) f[k] = p[cc + k];
             }
         }
         uint2 u;
-        u.x = (unsigned)f2bf(f.x) | ((unsigned)f2bf(f.y) << 16);
-        u.y = (unsigned)f2bf(f.z) | ((unsigned)f2bf(f.w) << 16);
-        *reinterpret_cast<uint2 *>(&halo[hv * cst + c]) = u;
+        u.x = (unsigned)f2bf(f[0]) | ((unsigned)f2bf(f[1]) << 16);
+        u.y = (unsigned)f2bf(f[2]) | ((unsigned)f2bf(f[3]) << 16);
+        *reinterpret_cast<uint2 *>(&halo[hv * CST + c]) = u;
     }
     __syncthreads();
 
     const int m = lane & 15, q = lane >> 4;
-    const int vx = m & 3, vy = m >> 2, vz = w;
-    const int coutp = 16 * NT;
-    f4v acc[NT];
+    // halo index of this lane's voxel in each of its m-tiles (tap offset 0)
+    int hb[MT];
 #pragma unroll
-    for (int j = 0; j < NT; j++) acc[j] = f4v{0.f, 0.f, 0.f, 0.f};
-    const int K16 = (TAPS * cinp + 15) >> 4;
-    int tap = (4 * q) / cinp, c = 4 * q - tap * cinp;
-    for (int ks = 0; ks < K16; ks++) {
-        const int tp = tap < TAPS ? tap : 0;  // padding tap: zero weights
-        const int tx = tp % KS, ty = (tp / KS) % KS, tz = tp / (KS * KS);
-        const int hv = (vx + tx) + H * ((vy + ty) + H * (vz + tz));
-        const s4v a = *reinterpret_cast<const s4v *>(&halo[hv * cst + c]);
-        const unsigned short *wb = Wp + ((long long)tap * coutp + m) * cinp + c;
+    for (int t = 0; t < MT; t++) {
+        const int mt = w + 4 * t;
+        const int vx = 4 * (mt >> 2) + (m & 3), vy = m >> 2, vz = mt & 3;
+        hb[t] = vx + HX * (vy + H * vz);
+    }
+    // lane's tap and channel base within a tap group
+    auto tap_of = [&](int g) { return CINP >= 16 ? g : 2 * g + (q >> 1); };
+    const int cq = CINP >= 16 ? 4 * q : 4 * (q & 1);
+
+    f4v acc[MT][NT];
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const s4v b = *reinterpret_cast<const s4v *>(wb + (long long)j * 16 * cinp);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[t][j] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    s4v bcur[GS][NT], bnxt[GS][NT];
+    auto load_b = [&](int g, s4v (&b)[GS][NT]) {
+        const int tap = tap_of(g);
+#pragma unroll
+        for (int k = 0; k < GS; k++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                b[k][j] = *reinterpret_cast<const s4v *>(
+                    Wp + ((long long)(tap * COUTP + j * 16 + m)) * CINP + k * 16 + cq);
+    };
+    load_b(0, bcur);
+    for (int g = 0; g < NG; g++) {
+        if (g + 1 < NG) load_b(g + 1, bnxt);
+        int tap = tap_of(g);
+        if (tap >= TAPS) tap = 0;  // padding tap: its weights are zero
+        const int tx = tap % KS, ty = (tap / KS) % KS, tz = tap / (KS * KS);
+        const int toff = tx + HX * (ty + H * tz);
+#pragma unroll
+        for (int k = 0; k < GS; k++) {
+            s4v a[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+                a[t] = *reinterpret_cast<const s4v *>(&halo[(hb[t] + toff) * CST + k * 16 + cq]);
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bcur[k][j], acc[t][j], 0, 0, 0);
         }
-        c += 16;
-        while (c >= cinp) { c -= cinp; tap++; }
+#pragma unroll
+        for (int k = 0; k < GS; k++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) bcur[k][j] = bnxt[k][j];
     }
 
 #pragma unroll
-    for (int j = 0; j < NT; j++) {
-        const int n = j * 16 + m;
-        if (n >= cout) continue;
-        const float bn = bias ? bias[n] : 0.f;
+    for (int t = 0; t < MT; t++) {
+        const int mt = w + 4 * t;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int mo = 4 * q + r;
-            const int x = 4 * bx + (mo & 3), y = 4 * by + (mo >> 2), z = 4 * bz + vz;
-            if (x >= e || y >= e || z >= e) continue;
-            float v = acc[j][r] + bn;
-            if (relu) v = fmaxf(v, 0.f);
-            Y[((long long)s * V + x + (long long)e * (y + (long long)e * z)) * cout + n] = v;
+        for (int j = 0; j < NT; j++) {
+            const int n = j * 16 + m;
+            if (n >= cout) continue;
+            const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int mo = 4 * q + r;
+                const int x = BX * bx + 4 * (mt >> 2) + (mo & 3), y = 4 * by + (mo >> 2),
+                          z = 4 * bz + (mt & 3);
+                if (x >= e || y >= e || z >= e) continue;
+                float v = acc[t][j][r] + bn;
+                if (relu) v = fmaxf(v, 0.f);
+                Y[((long long)s * V + x + (long long)e * (y + (long long)e * z)) * cout + n] = v;
+            }
         }
     }
 }
@@ -493,29 +555,47 @@ void pack_conv_weights(hipStream_t st, const unsigned short *W16, unsigned short
     ST_HIP(hipGetLastError());
 }
 
-void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, const float *bias,
-                  float *Y, int ns, int e, int cin, int cout, int ksz, bool relu) {
+void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const float *X1, int C1,
+                   const unsigned short *Wp, const float *bias, float *Y, int ns, int e, int cout,
+                   int ksz, bool relu) {
+    const int cin = C0 + C1;
     ST_REQUIRE(ksz == 1 || ksz == 3, ST_ERR_ARG, "conv kernel size must be 1 or 3");
-    ST_REQUIRE(cout >= 1 && cout <= 64 && cin >= 1 && cin <= 256 && e >= 1, ST_ERR_ARG,
-               "conv shape out of range");
+    ST_REQUIRE(C0 >= 1 && C1 >= 0 && (C1 == 0 || X1) && cout >= 1 && cout <= 64 && e >= 1 &&
+                   (!up0 || e % 2 == 0),
+               ST_ERR_ARG, "conv shape out of range");
+    ST_REQUIRE(C1 == 0 || C0 % 4 == 0, ST_ERR_ARG, "concatenated inputs need C0 % 4 == 0");
     const int cinp = conv_cinp(cin), coutp = conv_coutp(cout);
-    const int H = 4 + ksz - 1;
-    const size_t lds = sizeof(unsigned short) * (size_t)H * H * H * (cinp + 4);
-    ST_REQUIRE(lds <= 65536, ST_ERR_ARG, "conv input halo exceeds 64 KiB of LDS");
-    const int nb = (e + 3) / 4;
-    const long long nwg = (long long)ns * nb * nb * nb;
+    const int bx = cinp <= 48 ? 8 : 4;
+    const long long nwg = (long long)ns * ((e + bx - 1) / bx) * ((e + 3) / 4) * ((e + 3) / 4);
     ST_REQUIRE(nwg < (1LL << 31), ST_ERR_ARG, "conv grid too large");
     if (nwg == 0) return;
     const dim3 g((unsigned)nwg), b(256);
-    const int r = relu ? 1 : 0;
-#define ST_CONV(KS, NT) hipLaunchKernelGGL((conv3d_fwd<KS, NT>), g, b, lds, st, X, Wp, bias, Y, e, cin, cinp, cout, r)
-    if (ksz == 3) {
-        if (coutp == 16) ST_CONV(3, 1); else if (coutp == 32) ST_CONV(3, 2); else ST_CONV(3, 4);
-    } else {
-        if (coutp == 16) ST_CONV(1, 1); else if (coutp == 32) ST_CONV(1, 2); else ST_CONV(1, 4);
+    const int u = up0 ? 1 : 0, r = relu ? 1 : 0;
+#define ST_CONV(KS, CP, NT, BXV)                                                                  \
+    if (ksz == KS && cinp == CP && coutp == 16 * NT) {                                            \
+        ST_REQUIRE(bx == BXV, ST_ERR_ARG, "conv brick");                                          \
+        hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV>), g, b, 0, st, X0, C0, u, X1, C1, Wp,     \
+                           bias, Y, e, cout, r);                                                  \
+        ST_HIP(hipGetLastError());                                                                \
+        return;                                                                                   \
     }
+    ST_CONV(3, 8, 1, 8)
+    ST_CONV(3, 16, 1, 8)
+    ST_CONV(3, 48, 1, 8)
+    ST_CONV(3, 16, 2, 8)
+    ST_CONV(3, 32, 2, 8)
+    ST_CONV(3, 96, 2, 4)
+    ST_CONV(3, 32, 4, 8)
+    ST_CONV(3, 64, 4, 4)
+    ST_CONV(3, 128, 4, 4)
+    ST_CONV(1, 16, 1, 8)
 #undef ST_CONV
-    ST_HIP(hipGetLastError());
+    ST_REQUIRE(false, ST_ERR_ARG, "no convolution kernel for this channel configuration");
+}
+
+void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, const float *bias,
+                  float *Y, int ns, int e, int cin, int cout, int ksz, bool relu) {
+    conv_forward2(st, X, cin, false, nullptr, 0, Wp, bias, Y, ns, e, cout, ksz, relu);
 }
 
 }  // namespace nn

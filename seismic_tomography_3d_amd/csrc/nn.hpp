@@ -55,6 +55,11 @@ void pack_conv_weights(hipStream_t st, const unsigned short *W16, unsigned short
                        int cin, int cout);
 void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, const float *bias,
                   float *Y, int ns, int e, int cin, int cout, int ksz, bool relu);
+// the same with the input the concatenation [X0 (C0 channels, nearest-
+// upsampled from half resolution when up0) | X1 (C1 channels)]
+void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const float *X1, int C1,
+                   const unsigned short *Wp, const float *bias, float *Y, int ns, int e, int cout,
+                   int ksz, bool relu);
 
 }  // namespace nn
 }  // namespace st

@@ -189,6 +189,15 @@ static void conv_fwd(st_ctx *ctx, Net &n, int li, const float *X, float *Y, int 
                      ns, n.nc / c.res_div, c.cin, c.cout, c.ksz, c.relu);
 }
 
+// decoder convolution on [upsample(Xup) (Cup channels) | Xskip (Cskip)]
+static void conv_cat(st_ctx *ctx, Net &n, int li, const float *Xup, int Cup, const float *Xskip,
+                     int Cskip, float *Y, int ns) {
+    const Conv &c = n.L[li];
+    ST_REQUIRE(Cup + Cskip == c.cin, ST_ERR_ARG, "concat channels do not match the layer");
+    nn::conv_forward2(ctx->stream, Xup, Cup, true, Xskip, Cskip, n.wp.as<unsigned short>() + c.op,
+                      n.w.as<float>() + c.ob, Y, ns, n.nc / c.res_div, c.cout, c.ksz, c.relu);
+}
+
 // re-packs the BF16 weights for conv3d_fwd after any change of n.w16
 static void net_pack(st_ctx *ctx, Net &n) {
     if (n.pver == n.wver) return;
@@ -243,7 +252,11 @@ static void add(st_ctx *ctx, float *y, const float *x, long long n) {
     hipLaunchKernelGGL(add_into, dim3((unsigned)b), dim3(256), 0, ctx->stream, y, x, n);
 }
 
-static void net_forward(st_ctx *ctx, Net &n, int ns) {
+// Forward pass.  With fused, the decoder convolutions read the upsampled
+// deeper activation and the skip activation directly (no concat buffers);
+// the backward pass needs the materialised concats, so training and
+// gradient evaluation run unfused.
+static void net_forward(st_ctx *ctx, Net &n, int ns, bool fused = false) {
     hipStream_t st = ctx->stream;
     const int e1 = n.nc, e2 = e1 / 2, e4 = e1 / 4, e8 = e1 / 8;
     net_pack(ctx, n);
@@ -256,15 +269,21 @@ static void net_forward(st_ctx *ctx, Net &n, int ns) {
     conv_fwd(ctx, n, 4, n.p2.as<float>(), n.a5.as<float>(), ns);
     nn::pool(st, n.a5.as<float>(), n.p3.as<float>(), ns, e4, 64);
     conv_fwd(ctx, n, 5, n.p3.as<float>(), n.a7.as<float>(), ns);
-    nn::upsample(st, n.a7.as<float>(), n.cat3.as<float>(), ns, e8, 64, 128, 0);
-    nn::chancopy(st, n.a5.as<float>(), n.cat3.as<float>(), cube(e4) * ns, 64, 128, 64, false);
-    conv_fwd(ctx, n, 6, n.cat3.as<float>(), n.a8.as<float>(), ns);
-    nn::upsample(st, n.a8.as<float>(), n.cat2.as<float>(), ns, e4, 64, 96, 0);
-    nn::chancopy(st, n.a4.as<float>(), n.cat2.as<float>(), cube(e2) * ns, 32, 96, 64, false);
-    conv_fwd(ctx, n, 7, n.cat2.as<float>(), n.a9.as<float>(), ns);
-    nn::upsample(st, n.a9.as<float>(), n.cat1.as<float>(), ns, e2, 32, 48, 0);
-    nn::chancopy(st, n.a2.as<float>(), n.cat1.as<float>(), cube(e1) * ns, 16, 48, 32, false);
-    conv_fwd(ctx, n, 8, n.cat1.as<float>(), n.a10.as<float>(), ns);
+    if (fused) {
+        conv_cat(ctx, n, 6, n.a7.as<float>(), 64, n.a5.as<float>(), 64, n.a8.as<float>(), ns);
+        conv_cat(ctx, n, 7, n.a8.as<float>(), 64, n.a4.as<float>(), 32, n.a9.as<float>(), ns);
+        conv_cat(ctx, n, 8, n.a9.as<float>(), 32, n.a2.as<float>(), 16, n.a10.as<float>(), ns);
+    } else {
+        nn::upsample(st, n.a7.as<float>(), n.cat3.as<float>(), ns, e8, 64, 128, 0);
+        nn::chancopy(st, n.a5.as<float>(), n.cat3.as<float>(), cube(e4) * ns, 64, 128, 64, false);
+        conv_fwd(ctx, n, 6, n.cat3.as<float>(), n.a8.as<float>(), ns);
+        nn::upsample(st, n.a8.as<float>(), n.cat2.as<float>(), ns, e4, 64, 96, 0);
+        nn::chancopy(st, n.a4.as<float>(), n.cat2.as<float>(), cube(e2) * ns, 32, 96, 64, false);
+        conv_fwd(ctx, n, 7, n.cat2.as<float>(), n.a9.as<float>(), ns);
+        nn::upsample(st, n.a9.as<float>(), n.cat1.as<float>(), ns, e2, 32, 48, 0);
+        nn::chancopy(st, n.a2.as<float>(), n.cat1.as<float>(), cube(e1) * ns, 16, 48, 32, false);
+        conv_fwd(ctx, n, 8, n.cat1.as<float>(), n.a10.as<float>(), ns);
+    }
     conv_fwd(ctx, n, 9, n.a10.as<float>(), n.out.as<float>(), ns);
     ST_HIP(hipGetLastError());
 }
@@ -480,7 +499,7 @@ bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float
         const int ns = std::min(chunk, nshot - s0);
         hipLaunchKernelGGL(prep_inputs, dim3(nblocks(V * ns)), dim3(256), 0, ctx->stream,
                            ctx->slow_zxy.as<float>(), dsrc + 3LL * s0, ns, g, n.in0.as<float>());
-        net_forward(ctx, n, ns);
+        net_forward(ctx, n, ns, true);
         ST_HIP(hipMemcpyAsync(n.Q.as<float>() + s0 * V, n.out.ptr, sizeof(float) * V * ns,
                               hipMemcpyDeviceToDevice, ctx->stream));
     }
@@ -564,7 +583,7 @@ int st_surrogate_eval(st_ctx *ctx, const float *X, int ns, const float *target, 
         net_alloc_batch(n, ns, target && mask);
         const long long V = cube(n.nc) * ns;
         ST_HIP(hipMemcpyAsync(n.in0.ptr, X, sizeof(float) * V * kIn, hipMemcpyHostToDevice, ctx->stream));
-        net_forward(ctx, n, ns);
+        net_forward(ctx, n, ns, !(target && mask));
         if (Y) ST_HIP(hipMemcpyAsync(Y, n.out.ptr, sizeof(float) * V, hipMemcpyDeviceToHost, ctx->stream));
         if (target && mask) {
             n.bY.reserve(sizeof(float) * V);
@@ -822,7 +841,7 @@ int st_surrogate_predict(st_ctx *ctx, const float *src_xyz, int nshot, float *y_
         hipLaunchKernelGGL(prep_inputs, dim3(nblocks(V * nshot)), dim3(256), 0, ctx->stream,
                            ctx->slow_zxy.as<float>(), ctx->src.as<float>(), nshot,
                            cube_geo(ctx, n.nc), n.in0.as<float>());
-        net_forward(ctx, n, nshot);
+        net_forward(ctx, n, nshot, true);
         ST_HIP(hipMemcpyAsync(y_out, n.out.ptr, sizeof(float) * V * nshot, hipMemcpyDeviceToHost,
                               ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
