@@ -111,6 +111,20 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
     return base, err
 
 
+def calibrate_copy(mib):
+    """A device-to-device copy of known size (torch's copy kernel), three
+    times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch
+    give the scale of those counters against algorithmic bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "dispatches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +140,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--check-shots", type=int, default=8)
+    ap.add_argument("--band", type=float, default=0.0,
+                    help="warm-start ordering band in s (0 = tuned)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-shot-at-a-time leg")
+    ap.add_argument("--no-cold-leg", action="store_true",
+                    help="skip the batched solve without the network")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB three times (HBM counter scale)")
     a = ap.parse_args()
 
     if a.gpus != 1:
@@ -133,6 +154,10 @@ def main():
                                    "(shots shard across GPUs by design: one shot "
                                    "range per rank, no collective in the solve)"}))
         sys.exit(2)
+
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
 
     from seismic_tomography_3d_amd import Context, synth, tune
 
@@ -156,7 +181,8 @@ def main():
                      max_inner=params["max_inner"], batch=a.batch)
     opts = dict(cold_opts)
     if warm:
-        wp = tune.best_warm_params(ctx, src, rcv, cold_opts, log=log)
+        wp = (dict(band=a.band, max_inner=params["max_inner"]) if a.band > 0
+              else tune.best_warm_params(ctx, src, rcv, cold_opts, log=log))
         opts.update(wp, warm=1)
     log(f"options {opts}")
     for _ in range(a.warmup):
@@ -172,7 +198,7 @@ def main():
 
     # the same batched solver without the network (what the network gains)
     batched_cold = None
-    if warm:
+    if warm and not a.no_cold_leg:
         ctx.fim_solve(src, rcv, **cold_opts)
         t0 = time.perf_counter()
         _, _, cst = ctx.fim_solve(src, rcv, **cold_opts)
@@ -184,34 +210,47 @@ def main():
 
     # plain FIM: same kernels, cold start, one shot at a time, its own tuned
     # launch shape (bounded sample)
-    npl = min(a.plain_shots, len(src))
-    pparams = tune.best_params(ctx, src[:npl], rcv, block=a.block, batch=1, log=log,
-                               candidates=[dict(block=B, grid=g, max_inner=16, batch=1)
-                                           for B in (4, 8) for g in (256, 512, 1024, 2048)])
-    popts = dict(pparams, batch=1, warm=0)
-    ctx.fim_solve(src[:2], rcv, **popts)
-    t0 = time.perf_counter()
-    _, _, pst = ctx.fim_solve(src[:npl], rcv, **popts)
-    pdt = time.perf_counter() - t0
-    plain = {"value": npl / pdt, "unit": "shot-solves/s", "shots": npl,
-             "ms_per_shot": 1e3 * pdt / npl, "rounds_per_shot": pst["rounds"] / npl,
-             "visits_per_shot": pst["visits"] / npl, "options": popts}
-    log(f"plain: {plain}")
+    plain = None
+    if not a.no_plain:
+        npl = min(a.plain_shots, len(src))
+        pparams = tune.best_params(ctx, src[:npl], rcv, block=a.block, batch=1, log=log,
+                                   candidates=[dict(block=B, grid=g, max_inner=mi, batch=1)
+                                               for B in (4, 8) for g in (1024, 2048)
+                                               for mi in (16, 24)])
+        popts = dict(pparams, batch=1, warm=0)
+        ctx.fim_solve(src[:2], rcv, **popts)
+        t0 = time.perf_counter()
+        _, _, pst = ctx.fim_solve(src[:npl], rcv, **popts)
+        pdt = time.perf_counter() - t0
+        plain = {"value": npl / pdt, "unit": "shot-solves/s", "shots": npl,
+                 "ms_per_shot": 1e3 * pdt / npl, "rounds_per_shot": pst["rounds"] / npl,
+                 "visits_per_shot": pst["visits"] / npl, "options": popts}
+        log(f"plain: {plain}")
 
     # roofline of the FIM round kernel over the last timed step
     B = params["block"]
     bpv = bytes_per_visit(B)
-    achieved = stats["visits"] * bpv / (stats["solve_ms"] / 1e3) / 1e9
+    # per launch: algorithmic bytes of the visits one round makes, over the
+    # average round duration (HIP events around the rounds on the library's
+    # stream, divided by the launches)
+    launches = max(1, stats["rounds"])
+    avg_launch_s = stats["solve_ms"] / 1e3 / launches
+    bytes_per_launch = stats["visits"] * bpv / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": f"fim_round{B}", "bytes_per_visit": bpv,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_s * 1e6,
             "visits_per_step": stats["visits"], "kernel_ms_per_step": stats["solve_ms"],
             "launches_per_step": stats["rounds"]}
     prof = os.path.join(ROOT, "profiles", f"fim_round{B}_batched_201x201x101_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             tr = json.load(f)
-        roof["traffic"] = tr.get("bytes_per_visit_measured")
+        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counters of
+        # the same command, scaled by their readings for a copy of known size
+        roof["traffic"] = tr.get("bytes_per_launch")
+        roof["traffic_source"] = os.path.relpath(prof, ROOT)
 
     out = {
         "metric": METRIC, "value": value, "unit": "shot-solves/s", "n_gpus": 1,
@@ -224,7 +263,7 @@ def main():
                    "max_inner": params["max_inner"], "warm_start": bool(warm),
                    "options": opts},
         "plain_fim": plain,
-        "speedup_vs_plain": value / plain["value"],
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "batched_cold_fim": batched_cold,
         "network_gain": (value / batched_cold["value"]) if batched_cold else None,
         "fim_stats_per_step": stats,

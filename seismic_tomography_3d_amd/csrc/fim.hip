@@ -407,11 +407,18 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round8(FimArgs A) {
 
         int it = 0;
         bool more = true;
+        // dirty: z planes whose inputs changed since their last update.  A
+        // plane reads its own plane (x/y neighbours, Jacobi) and planes k - 1
+        // and k + 1 (z neighbours); the halos are fixed for the visit.  A
+        // plane none of whose inputs changed reproduces its values, so it is
+        // skipped (wave-uniform), and the visit ends when no plane is dirty.
+        unsigned dirty = 0xFFu;
         while (it < A.max_inner) {
-            bool ch = false;
             // one plane of the column: x/y neighbours from the tile (Jacobi
             // across lanes), z neighbours from registers (Gauss-Seidel)
             auto plane = [&](const int k) {
+                if (!((dirty >> k) & 1u)) return;
+                dirty &= ~(1u << k);
                 const int ti = tcol + 100 * (k + 1);
                 const float a = fminf(tile[ti - 1], tile[ti + 1]);
                 const float bb = fminf(tile[ti - 10], tile[ti + 10]);
@@ -419,11 +426,12 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round8(FimArgs A) {
                 const float t = godunov(a, bb, c, hs[k], hs[k] * hs[k]);
                 const bool valid = vxy && (bz * 8 + k < A.n4z);
                 const float nv = valid ? fminf(vz[k + 1], t) : vz[k + 1];
-                ch |= (vz[k + 1] - nv) > A.eps;
+                const bool ch = (vz[k + 1] - nv) > A.eps;
                 vz[k + 1] = nv;
                 wave_sync();
                 tile[ti] = nv;
                 wave_sync();
+                if (__ballot(ch)) dirty |= ((7u << k) >> 1) & 0xFFu;
             };
             if ((it & 1) == 0) {
                 #pragma unroll
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round8(FimArgs A) {
                 for (int k = 7; k >= 0; k--) plane(k);
             }
             ++it;
-            more = __ballot(ch) != 0ull;
+            more = dirty != 0u;
             if (!more) break;
         }
         float v[8];
