@@ -34,15 +34,18 @@ typedef struct st_ctx st_ctx;
 
 /* Eikonal solver options (block Fast Iterative Method). */
 typedef struct {
-    int block;          /* block edge: 4 or 8 (0 = tuned default)             */
-    int batch;          /* shots per persistent solve; 1 = one shot at a time  */
-    int max_inner;      /* inner iteration cap per block visit (0 = default)   */
-    float eps;          /* change threshold in seconds (<0 = default 0)        */
+    int block;          /* block edge: 4 or 8 (default 8)                      */
+    int batch;          /* shots per persistent solve (default 256); 1 = one
+                           shot at a time                                      */
+    int max_inner;      /* inner iteration cap per block visit (0 = 24 for
+                           8^3 blocks, 16 for 4^3)                             */
+    float eps;          /* change threshold in seconds (<0 = default 1e-6, the
+                           reference's EPS; 0 = iterate to the exact fixed point) */
     int warm;           /* 0 cold start, 1 surrogate warm start                */
     float delta;        /* warm start inflation (1+delta) (0 = 0.02)           */
     int judge;          /* 1: verify shots started from seeded values
                            (warm_values = 1), re-solve cold on failure         */
-    int grid;           /* waves per round launch (0 = auto)                   */
+    int grid;           /* waves per round launch (0 = 32 per CU)              */
     int rounds_per_sync;/* rounds enqueued between host checks (0 = auto)      */
     int use_graph;      /* 1: replay rounds from a captured hipGraph           */
     float band;         /* warm starts: visit a block only once its predicted

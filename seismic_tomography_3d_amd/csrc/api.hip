@@ -82,8 +82,8 @@ extern "C" {
 void st_fim_default_opts(st_fim_opts *o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));
-    o->block = 4;
-    o->batch = 64;
+    o->block = 8;
+    o->batch = 256;
     o->max_inner = 0;
     o->eps = -1.0f;
     o->warm = 0;
@@ -192,8 +192,8 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
     st_fim_opts o;
     if (opts) o = *opts; else st_fim_default_opts(&o);
     ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
-    if (o.max_inner <= 0) o.max_inner = 16;
-    if (o.eps < 0.f) o.eps = 0.f;
+    if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 24 : 16;
+    if (o.eps < 0.f) o.eps = kFimEps;
     if (o.delta <= 0.f) o.delta = 0.02f;
     ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
     const bool warm = (o.warm == 1) || T_pred;
@@ -419,8 +419,8 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
         st_fim_opts o;
         if (opts) o = *opts; else st_fim_default_opts(&o);
         ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
-        if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 16 : 16;
-        if (o.eps < 0.f) o.eps = 0.f;
+        if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 24 : 16;
+        if (o.eps < 0.f) o.eps = kFimEps;
         ST_REQUIRE(o.warm == 0, ST_ERR_ARG, "forward runs cold starts");
         if (stats) std::memset(stats, 0, sizeof(*stats));
         check_points(src_xyz, nshot, ctx, "source");

@@ -814,7 +814,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
 
     // o.grid counts waves; a launch has o.grid / kWaves workgroups
     int grid = o.grid;
-    if (grid <= 0) grid = 256 * 16;
+    if (grid <= 0) grid = 256 * 32;  // 32 waves per CU
     grid = std::max(1, (grid + kWaves - 1) / kWaves);
     const int nwaves = grid * kWaves;
     if (stats) {

@@ -15,6 +15,11 @@
 namespace st {
 
 constexpr float kInf = 1e6f;  // "not reached" (cuda_kernel.cuh:7)
+// default FIM change threshold in seconds: a cell counts as changed (and
+// its block as unconverged) when it decreased by more than this -- the
+// reference's block-FIM tolerance EPS (cuda_kernel.cuh:8, used at
+// cuda_kernel.cu:414)
+constexpr float kFimEps = 1e-6f;
 
 // Thrown inside the library, turned into a status at the C boundary.
 struct Error {

@@ -122,4 +122,14 @@ def test_training_and_warm_start_gain(ctx, tmp_path):
           f"fallbacks {sv['fallbacks']}/32 judge_max {sv['judge_max']:.2e} max err {errv:.2e}")
     assert err < 1e-5 and errv < 1e-5
     assert sw["fallbacks"] == 0
-    assert sw["inner_iters"] < sc["inner_iters"]
+    # the gain: with a visit cap below what a block needs to converge, cold
+    # solves revisit blocks; visiting them in predicted-arrival order removes
+    # most revisits (at this coarse grid an uncapped 8^3 visit already
+    # converges its block, the full-size case is measured by bench.py)
+    capped = dict(batch=32, block=8, max_inner=8)
+    _, Tc8, sc8 = ctx.fim_solve(src, rcv, return_T=True, **capped)
+    _, Tw8, sw8 = ctx.fim_solve(src, rcv, return_T=True, warm=1, **capped)
+    print(f"max_inner 8: cold iters {sc8['inner_iters']} visits {sc8['visits']}; ordered iters "
+          f"{sw8['inner_iters']} visits {sw8['visits']}")
+    assert np.abs(Tw8 - Tc8).max() < 1e-5
+    assert sw8["inner_iters"] < 0.7 * sc8["inner_iters"]
