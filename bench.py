@@ -230,10 +230,11 @@ def main():
     # roofline of the FIM round kernel over the last timed step
     B = params["block"]
     bpv = bytes_per_visit(B)
-    # per launch: algorithmic bytes of the visits one round makes, over the
-    # average round duration (HIP events around the rounds on the library's
-    # stream, divided by the launches)
-    launches = max(1, stats["rounds"])
+    # per launch: algorithmic bytes of the visits of one step over its round
+    # launches (the empty launches enqueued before the host saw the end
+    # included, as a kernel trace counts them), over the average launch
+    # duration (HIP events around the rounds on the library's stream)
+    launches = max(1, stats["launches"])
     avg_launch_s = stats["solve_ms"] / 1e3 / launches
     bytes_per_launch = stats["visits"] * bpv / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -242,7 +243,7 @@ def main():
             "kernel": f"fim_round{B}", "bytes_per_visit": bpv,
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_s * 1e6,
             "visits_per_step": stats["visits"], "kernel_ms_per_step": stats["solve_ms"],
-            "launches_per_step": stats["rounds"]}
+            "launches_per_step": stats["launches"], "rounds_per_step": stats["rounds"]}
     prof = os.path.join(ROOT, "profiles", f"fim_round{B}_batched_201x201x101_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:

@@ -70,6 +70,8 @@ typedef struct {
     double judge_max;       /* largest warm-start residual seen (s)       */
     double solve_ms;        /* device time of the FIM rounds              */
     double total_ms;        /* wall time of the call                      */
+    long long launches;     /* round kernels launched (rounds + the empty
+                               ones enqueued before the host saw the end) */
 } st_fim_stats;
 
 /* Fills opts with defaults. */

@@ -38,7 +38,7 @@ class FimStats(ctypes.Structure):
                 ("inner_iters", ctypes.c_longlong), ("fallbacks", ctypes.c_longlong),
                 ("batches", ctypes.c_longlong), ("deferred", ctypes.c_longlong),
                 ("judge_max", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("launches", ctypes.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

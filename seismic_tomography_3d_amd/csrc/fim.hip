@@ -910,6 +910,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         ST_HIP(hipStreamSynchronize(ctx->stream));
         const unsigned long long *hs = reinterpret_cast<const unsigned long long *>(hcnt + 8);
         stats->rounds += hcnt[4];
+        stats->launches += r;
         stats->visits += (long long)hs[0];
         stats->inner_iters += (long long)hs[1];
         stats->deferred += (long long)hs[2];
