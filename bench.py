@@ -77,13 +77,15 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
         t.start()
     for t in ths:
         t.join()
-    log(f"fp64 check of {len(sample_idx)} shots took {time.time() - t0:.1f}s")
+    t_fp64 = time.time() - t0
+    log(f"fp64 check of {len(sample_idx)} shots took {t_fp64:.1f}s")
     err = {
         "shots_checked": [int(i) for i in sample_idx],
         "max_pick_err_s": float(max(e[0] for e in errs)),
         "rms_pick_err_s": float(np.sqrt(np.mean([e[1] ** 2 for e in errs]))),
         "max_volume_err_s": float(max(e[2] for e in errs)),
         "per_shot_max_pick_err_s": [float(e[0]) for e in errs],
+        "fp64_oracle_s_per_shot": t_fp64 * min(threads, len(sample_idx)) / len(sample_idx),
     }
 
     # CPU reference: the reference's own Accurate_FSM when built, else the port
@@ -99,6 +101,16 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
         orc.oracle().or_fsm_batch(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), n, orc.P(rx),
                                   orc.P(ry), orc.P(rz), nr, orc.P(picks), threads)
     dt = time.time() - t0
+    # the same on one core, one shot
+    p1 = np.zeros((1, nr), np.float32)
+    t1 = time.time()
+    if kind == "reference":
+        orc.ref().ref_fsm_threads(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub[:1]), 1, orc.P(rx),
+                                  orc.P(ry), orc.P(rz), nr, orc.P(p1), 1)
+    else:
+        orc.oracle().or_fsm_batch(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub[:1]), 1, orc.P(rx),
+                                  orc.P(ry), orc.P(rz), nr, orc.P(p1), 1)
+    dt1 = time.time() - t1
     base = {
         "value": n / dt,
         "unit": "shot-solves/s",
@@ -107,6 +119,7 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
         "sample": f"{n} of the {len(src)} shots (one per thread), Accurate_FSM "
                   f"(Noble-Gesret-Belayouni, the reference's default CPU eikonal) + picks, "
                   f"201x201x101, {dt:.1f} s wall",
+        "one_core": {"value": 1.0 / dt1, "unit": "shot-solves/s", "sample": "shot 0, 1 thread"},
     }
     return base, err
 
@@ -269,7 +282,18 @@ def main():
         "network_gain": (value / batched_cold["value"]) if batched_cold else None,
         "fim_stats_per_step": stats,
         "roofline": roof,
+        # every shot must at least write its volume once and read the model:
+        # 8 B per cell, the HBM floor of any solver of this grid
+        "compulsory_floor": {"bytes_per_shot": 8 * int(np.prod(case["vel"].shape)),
+                             "shot_solves_per_s_at_8TBps":
+                                 HBM_PEAK_GBS * 1e9 / (8 * int(np.prod(case["vel"].shape)))},
     }
+    if batched_cold:
+        out["warm_start"] = {
+            "visit_reduction": batched_cold["visits"] / max(1, stats["visits"]),
+            "iter_reduction": batched_cold["inner_iters"] / max(1, stats["inner_iters"]),
+            "fallbacks": stats["fallbacks"],
+        }
 
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)

@@ -17,7 +17,7 @@ _CACHE = {}
 CUS = 256  # MI355X compute units
 
 
-def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=64, log=None,
+def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=256, log=None,
                 candidates=None, batch=None):
     key = (ctx.shape, len(src), block, grid, max_inner, batch,
            None if candidates is None else tuple(tuple(sorted(c.items())) for c in candidates))
@@ -62,7 +62,7 @@ def load_surrogate(ctx, case, log=None):
     return surrogate.attach(ctx, case, log=log)
 
 
-def best_warm_params(ctx, src, rcv, cold, sample=64, log=None):
+def best_warm_params(ctx, src, rcv, cold, sample=256, log=None):
     """Band of the predicted-arrival ordering and inner cap for warm solves,
     timed over the whole call (network inference included)."""
     n = min(sample, len(src))
@@ -73,7 +73,7 @@ def best_warm_params(ctx, src, rcv, cold, sample=64, log=None):
     unit = cold["block"] * h / 1500.0
     best, best_ms = None, float("inf")
     for mi in sorted({cold["max_inner"], 24}):
-        for f in (1.0, 1.5, 2.0, 3.0):
+        for f in (1.0, 1.5, 2.0):
             o = dict(cold, batch=n, warm=1, band=f * unit, max_inner=mi)
             ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
             _, _, st = ctx.fim_solve(sub, rcv, **o)
