@@ -116,6 +116,18 @@ int st_fim_solve_warm(st_ctx *ctx, const float *src_xyz, int nshot,
                       const float *T_pred, const st_fim_opts *opts,
                       st_fim_stats *stats);
 
+/* The reference's CPU eikonal, Accurate_FSM (Noble-Gesret-Belayouni
+ * operators: accurate_FSM.cpp:577-650 eikonal_equation with
+ * write_first_arrival :528-556), on the device for all shots of the call,
+ * bit-identical to the CPU code on the same model (isotropic or not).
+ * Replaces Accurate_FSM::eikonal_equation + write_first_arrival per shot
+ * (selected by eikonal_type, eikonal_main.cpp:15-22).  picks[nshot x nrcv];
+ * T_out (nullable) [nshot x ny x nx x nz], z fastest; batch = shots per
+ * device pass (0 = 256); device_ms (nullable) = device time of the sweeps. */
+int st_fsm_solve(st_ctx *ctx, const float *src_xyz, int nshot,
+                 const float *rcv_xyz, int nrcv, float *picks, float *T_out,
+                 int batch, double *device_ms);
+
 /* Receiver picks on a caller volume: write_first_arrival's trilinear
  * (block_FIM.cpp:228-262, trilinear.cpp:3-20) on the device. */
 int st_picks_volume(st_ctx *ctx, const float *T, int nx, int ny, int nz,

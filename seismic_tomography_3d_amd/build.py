@@ -34,12 +34,12 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 # files whose results must be bit-identical to the CPU formulas
-NO_CONTRACT = {"picks.hip", "tomo.hip"}
+NO_CONTRACT = {"picks.hip", "tomo.hip", "fsm.hip"}
 # files whose kernels never form or receive a NaN (inputs are checked or
 # sanitised on the bits); lets min/max skip quieting their operands
 NO_NANS = {"fim.hip"}
 
-HIP_SOURCES = ["api.hip", "fim.hip", "picks.hip", "tomo.hip", "surrogate.hip", "nn.hip"]
+HIP_SOURCES = ["api.hip", "fim.hip", "picks.hip", "tomo.hip", "surrogate.hip", "nn.hip", "fsm.hip"]
 HOST_SOURCES = ["file_manager.cpp", "geometry.cpp", "block_fim.cpp", "least_squares.cpp"]
 
 

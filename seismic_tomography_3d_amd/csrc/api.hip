@@ -13,6 +13,11 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
 void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nshot, int B,
                     float delta, float r_cold, bool values, float *dkeys);
 void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res);
+double fsm_batch(st_ctx *ctx, const float *dsrc, const float *hsrc, int nshot, const float *S);
+void fsm_pad(st_ctx *ctx, float *S);
+void fsm_picks(st_ctx *ctx, int nshot, const float *drcv, int nrcv, float *dpicks);
+void fsm_crop_volumes(st_ctx *ctx, float *dout, int nshot);
+int fsm_lanes(int nshot);
 void launch_extract(st_ctx *ctx, float *dout, int nshot, int B);
 void launch_picks_blk(hipStream_t st, const float *T, long long vol, int nshot, const float *rcv,
                       int nr, float *out, int nx, int ny, int nz, float dx, float dy, float dz,
@@ -130,7 +135,8 @@ int st_ctx_destroy(st_ctx *ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (auto *b : {&ctx->slow_zxy, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
-                    &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->wstats, &ctx->src, &ctx->rcv,
+                    &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->wstats, &ctx->fsm_S,
+                    &ctx->fsm_sid, &ctx->src, &ctx->rcv,
                     &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred})
         b->release();
     for (auto &e : ctx->ev)
@@ -161,6 +167,7 @@ int st_set_model(st_ctx *ctx, const float *slowness, int nx, int ny, int nz, flo
         ST_HIP(hipMemcpyAsync(ctx->slow_zxy.ptr, slowness, sizeof(float) * n,
                               hipMemcpyHostToDevice, ctx->stream));
         ctx->blk_valid[0] = ctx->blk_valid[1] = false;
+        ctx->fsm_valid = false;
         ctx->has_model = true;
         ctx->smin = *std::min_element(slowness, slowness + n);
         ctx->smax = *std::max_element(slowness, slowness + n);
@@ -339,6 +346,66 @@ int st_fim_solve_warm(st_ctx *ctx, const float *src_xyz, int nshot, const float 
         if (opts) o = *opts; else st_fim_default_opts(&o);
         o.warm = 0;  // the predictions come from the caller
         solve_impl(ctx, src_xyz, nshot, rcv_xyz, nrcv, picks, T_out, T_pred, &o, stats);
+    });
+}
+
+int st_fsm_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz, int nrcv,
+                 float *picks, float *T_out, int batch, double *device_ms) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(nshot >= 0 && nrcv >= 0, ST_ERR_ARG, "negative count");
+        ST_REQUIRE(nshot == 0 || src_xyz, ST_ERR_ARG, "null sources");
+        ST_REQUIRE(nrcv == 0 || (rcv_xyz && picks), ST_ERR_ARG, "null receivers or picks");
+        if (device_ms) *device_ms = 0.0;
+        if (nshot == 0) return;
+        check_points(src_xyz, nshot, ctx, "source");
+        check_points(rcv_xyz, nrcv, ctx, "receiver");
+        set_device(ctx);
+        const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+        const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+        if (!ctx->fsm_valid) {
+            ctx->fsm_S.reserve(sizeof(float) * nB);
+            fsm_pad(ctx, ctx->fsm_S.as<float>());
+            ctx->fsm_valid = true;
+        }
+        int b = batch > 0 ? std::min({batch, nshot, 1023}) : std::min(nshot, 256);
+        size_t free_b = 0, total_b = 0;
+        ST_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t per_shot = sizeof(float) * nB + (T_out ? sizeof(float) * npts : 0);
+        const size_t have = free_b + ctx->T.bytes + ctx->scratch.bytes;
+        while (b > 1 && per_shot * (size_t)fsm_lanes(b) > have / 2) b /= 2;
+        ctx->T.reserve(sizeof(float) * nB * fsm_lanes(b));
+        if (T_out) ctx->scratch.reserve(sizeof(float) * npts * b);
+        ctx->src.reserve(sizeof(float) * 3 * nshot);
+        ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
+        ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
+        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
+                              hipMemcpyHostToDevice, ctx->stream));
+        if (nrcv)
+            ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
+                                  hipMemcpyHostToDevice, ctx->stream));
+        double ms = 0.0;
+        for (int b0 = 0; b0 < nshot; b0 += b) {
+            const int nb = std::min(b, nshot - b0);
+            ms += fsm_batch(ctx, ctx->src.as<float>() + 3LL * b0, src_xyz + 3LL * b0, nb,
+                            ctx->fsm_S.as<float>());
+            if (nrcv)
+                fsm_picks(ctx, nb, ctx->rcv.as<float>(), nrcv,
+                          ctx->picks.as<float>() + (long long)b0 * nrcv);
+            if (T_out) {
+                fsm_crop_volumes(ctx, ctx->scratch.as<float>(), nb);
+                ST_HIP(hipMemcpyAsync(T_out + (long long)b0 * npts, ctx->scratch.ptr,
+                                      sizeof(float) * npts * nb, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                ST_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        }
+        if (nrcv)
+            ST_HIP(hipMemcpyAsync(picks, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+        if (device_ms) *device_ms = ms;
     });
 }
 

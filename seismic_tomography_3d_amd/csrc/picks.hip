@@ -17,6 +17,13 @@ struct ZxyLayout {  // caller volume, z fastest
     }
 };
 
+struct PadLayout {  // sweeping solver: one-cell padded volume, z fastest, NS shots per cell
+    int nxx, nzz, NS;
+    __device__ long long operator()(int x, int y, int z) const {
+        return ((z + 1) + (long long)(x + 1) * nzz + (long long)(y + 1) * nxx * nzz) * NS;
+    }
+};
+
 struct BlkLayout {  // solver block layout
     int B, nbx, nby;
     __device__ long long operator()(int x, int y, int z) const {
@@ -89,6 +96,17 @@ void launch_picks_zxy(hipStream_t st, const float *T, long long vol, int nshot, 
     if (n == 0) return;
     hipLaunchKernelGGL(picks_kernel<ZxyLayout>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        T, vol, nshot, rcv, nr, out, nx, ny, nz, dx, dy, dz, ZxyLayout{nx, nz});
+    ST_HIP(hipGetLastError());
+}
+
+void launch_picks_pad(hipStream_t st, const float *T, int NS, int nshot, const float *rcv, int nr,
+                      float *out, int nx, int ny, int nz, float dx, float dy, float dz) {
+    const long long n = (long long)nshot * nr;
+    if (n == 0) return;
+    // shot s of cell c is T[c * NS + s]: a shot's stride is 1
+    hipLaunchKernelGGL(picks_kernel<PadLayout>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       T, 1LL, nshot, rcv, nr, out, nx, ny, nz, dx, dy, dz,
+                       PadLayout{nx + 2, nz + 2, NS});
     ST_HIP(hipGetLastError());
 }
 

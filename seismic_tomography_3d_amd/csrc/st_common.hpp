@@ -127,6 +127,8 @@ struct st_ctx {
     st::DevBuf list[2];                // active lists (by round parity)
     st::DevBuf counters;               // list counts, stats
     st::DevBuf wstats;                 // per-wave FIM statistics slots
+    st::DevBuf fsm_S, fsm_sid;         // sweeping solver: padded slowness, source nodes
+    bool fsm_valid = false;            // fsm_S matches the model
     st::DevBuf src, rcv, picks, scratch;
     st::DevBuf keys, kmin, judge, pred;   // warm-start state
     void *host_pinned = nullptr;       // small pinned staging area
