@@ -40,7 +40,8 @@ NO_CONTRACT = {"picks.hip", "tomo.hip", "fsm.hip"}
 NO_NANS = {"fim.hip"}
 
 HIP_SOURCES = ["api.hip", "fim.hip", "picks.hip", "tomo.hip", "surrogate.hip", "nn.hip", "fsm.hip"]
-HOST_SOURCES = ["file_manager.cpp", "geometry.cpp", "block_fim.cpp", "least_squares.cpp"]
+HOST_SOURCES = ["file_manager.cpp", "geometry.cpp", "device_eikonal.cpp", "block_fim.cpp",
+                "accurate_fsm.cpp", "least_squares.cpp"]
 
 
 def _run(cmd):
