@@ -124,6 +124,107 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
     return base, err
 
 
+def fsm_leg(ctx, src, rcv, cpu_value):
+    """The reference's default CPU eikonal, Accurate_FSM, on the device
+    (bit-identical to the CPU code) over the same 256 shots."""
+    ctx.fsm_solve(src[:64], rcv)
+    t0 = time.perf_counter()
+    _, _, ms = ctx.fsm_solve(src, rcv)
+    dt = time.perf_counter() - t0
+    out = {"value": len(src) / dt, "unit": "shot-solves/s", "ms_per_step": 1e3 * dt,
+           "device_ms": ms}
+    if cpu_value:
+        out["speedup_vs_cpu_reference"] = out["value"] / cpu_value
+    return out
+
+
+def tomography_leg(with_cpu):
+    """One least-squares tomography iteration at the C2 configuration (16
+    shots, 2,500 receivers, 201x201x101 at 12.5 m, inversion grid 41x41x21,
+    order 2, lambda 1e-2, 20 CGLS iterations, 5-sample Gaussian): forward
+    (FIM + picks + ray tracing into the CSR matrix), Tikhonov CGLS and model
+    update on the device; beside it, the oracle's restatement of the
+    reference's host-serial steps (ray tracing per ray, FP32 COO CGLS, model
+    update with smoothing) on the same inputs, one core."""
+    from seismic_tomography_3d_amd import Context, synth
+    c = synth.tomography_case()
+    h = c["h"]
+    src = c["src"]
+    rcv = np.stack([c["rx"], c["ry"], c["rz"]], 1).astype(np.float32)
+    ctx = Context(0)
+    ctx.set_model(1.0 / c["vel_true"], h)
+    dobs, _, _ = ctx.fim_solve(src, rcv)
+    dobs = (dobs + np.random.default_rng(3).normal(0.0, 2e-3, dobs.shape)).astype(np.float32)
+    ctx.set_model(1.0 / c["vel0"], h)
+    ctx.tomo_set_grid(c["d_tomo"])
+    ctx.forward(src, rcv)  # warm-up (buffers)
+    ctx.set_model(1.0 / c["vel0"], h)
+    t0 = time.perf_counter()
+    dcal, _, fst = ctx.forward(src, rcv)
+    t1 = time.perf_counter()
+    ctx.optimize(dobs, dcal, 2, 1e-2, 20)
+    t2 = time.perf_counter()
+    ctx.model_update(None, smooth=True, samples=5, stdv=2.0)
+    t3 = time.perf_counter()
+    rp, cols, vals = ctx.csr()
+    out = {"workload": "16 shots x 2,500 receivers, 201x201x101 at 12.5 m, 41x41x21 inversion "
+                       "grid, order 2, lambda 1e-2, 20 CGLS iterations, Gaussian 5/2.0",
+           "gpu_ms": {"forward": 1e3 * (t1 - t0), "forward_fim_device": fst["solve_ms"],
+                      "optimize": 1e3 * (t2 - t1), "model_update": 1e3 * (t3 - t2),
+                      "iteration": 1e3 * (t3 - t0)},
+           "rays": int(src.shape[0] * rcv.shape[0]), "nnz_with_tikhonov": int(rp[-1])}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        o = orc.oracle()
+        _, T, _ = ctx.fim_solve(src, rcv, return_T=True)  # the iteration's volumes
+        ny, nx, nz = T.shape[1:]
+        nxt, nyt, nzt = ctx.tomo_dims()
+        dt_ = c["d_tomo"]
+        rx, ry, rz = (np.ascontiguousarray(rcv[:, i]) for i in range(3))
+        nr = rcv.shape[0]
+        cap = 4_000_000
+        rn = np.zeros(nr, np.int32)
+        cl = np.zeros(cap, np.int32)
+        vl = np.zeros(cap, np.float32)
+        ill = np.zeros((ny, nx, nz), np.float32)
+        Tp = np.zeros((ny + 2, nx + 2, nz + 2), np.float32)
+        t_ray = 0.0
+        for i, sxyz in enumerate(src):
+            o.or_expand_fdm(orc.P(np.ascontiguousarray(T[i])), nx, ny, nz, orc.P(Tp))
+            a = time.perf_counter()
+            o.or_ray_trace_shot(orc.P(Tp), nx, ny, nz, h, h, h, nxt, nyt, nzt, dt_, dt_, dt_,
+                                *(float(v) for v in sxyz), orc.P(rx), orc.P(ry), orc.P(rz), nr,
+                                100000, orc.P(rn), orc.P(cl), orc.P(vl), cap, orc.P(ill))
+            t_ray += time.perf_counter() - a
+        # the device system (data rows + Tikhonov rows) and right-hand side
+        M = nxt * nyt * nzt
+        N = rp.shape[0] - 1
+        b = np.zeros(N, np.float32)
+        b[:dobs.size] = (dobs - dcal).ravel()
+        x = np.zeros(M, np.float32)
+        a = time.perf_counter()
+        o.or_cgls_ref_f32(rp.ctypes.data_as(orc.lp), np.ascontiguousarray(cols).ctypes.data_as(orc.ip),
+                          orc.P(np.ascontiguousarray(vals)), N, M, orc.P(b), 20, orc.P(x))
+        t_cg = time.perf_counter() - a
+        sl = np.ascontiguousarray((1.0 / c["vel0"]).astype(np.float32))
+        dm = np.zeros_like(sl)
+        vel = np.zeros_like(sl)
+        a = time.perf_counter()
+        o.or_model_update(orc.P(sl), orc.P(x), nx, ny, nz, h, h, h, nxt, nyt, nzt, dt_, dt_, dt_,
+                          1, 5, 2.0, orc.P(dm), orc.P(vel))
+        t_up = time.perf_counter() - a
+        out["cpu_port_ms"] = {"ray_tracing": 1e3 * t_ray, "cgls": 1e3 * t_cg,
+                              "model_update": 1e3 * t_up, "cores": 1,
+                              "kind": "port (the oracle's restatement of least_squares.cpp)"}
+        g = out["gpu_ms"]
+        out["speedup_host_steps"] = (t_ray + t_cg + t_up) * 1e3 / max(
+            1e-9, g["forward"] - g["forward_fim_device"] + g["optimize"] + g["model_update"])
+    ctx.close()
+    return out
+
+
+
 def calibrate_copy(mib):
     """A device-to-device copy of known size (torch's copy kernel), three
     times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch
@@ -158,6 +259,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot-at-a-time leg")
     ap.add_argument("--no-cold-leg", action="store_true",
                     help="skip the batched solve without the network")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the Accurate_FSM and tomography-iteration legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (HBM counter scale)")
     a = ap.parse_args()
@@ -305,7 +408,14 @@ def main():
         out["speedup_vs_cpu"] = value / base["value"]
         out["error_vs_fp64"] = err
         out["error_gate_ok"] = err["max_pick_err_s"] <= 1e-4
+    if not a.no_extra:
+        cpu_v = out["cpu_baseline"]["value"] if "cpu_baseline" in out else None
+        out["accurate_fsm_gpu"] = fsm_leg(ctx, src, rcv, cpu_v)
+        log(f"accurate_fsm_gpu: {out['accurate_fsm_gpu']}")
     ctx.close()
+    if not a.no_extra:
+        out["tomography_iteration"] = tomography_leg(not a.no_cpu)
+        log(f"tomography_iteration: {out['tomography_iteration']}")
     print(json.dumps(out), flush=True)
 
 

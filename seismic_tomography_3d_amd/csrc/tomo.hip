@@ -20,6 +20,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "st_common.hpp"
 
 namespace st {
@@ -48,6 +50,7 @@ struct TomoState {
     int n_model = 0, n_data = 0, order = 0, N = 0, M = 0;
     long long annz = 0;
     DevBuf t_colptr, t_rows, t_vals, t_cursor;
+    DevBuf t_keys, t_idx0, t_idx1, t_sort;  // column-sort staging (radix sort pairs)
     DevBuf b, s, q, r, p, x, dots, scal;
     bool have_x = false;
 
@@ -64,7 +67,7 @@ void tomo_release(st_ctx *ctx) {
     TomoState &t = *static_cast<TomoState *>(ctx->tomo);
     for (DevBuf *b : {&t.rowptr, &t.cols, &t.vals, &t.runs, &t.nruns, &t.fdist, &t.rowlen,
                       &t.scanbuf, &t.err, &t.illum, &t.t_colptr, &t.t_rows, &t.t_vals,
-                      &t.t_cursor, &t.b, &t.s, &t.q, &t.r, &t.p, &t.x, &t.dots, &t.scal, &t.dm,
+                      &t.t_cursor, &t.t_keys, &t.t_idx0, &t.t_idx1, &t.t_sort, &t.b, &t.s, &t.q, &t.r, &t.p, &t.x, &t.dots, &t.scal, &t.dm,
                       &t.upd, &t.kern})
         b->release();
     delete &t;
@@ -380,46 +383,29 @@ __global__ void col_count(const int *__restrict__ cols, long long nnz, int *__re
     if (e < nnz) atomicAdd(&cnt[cols[e]], 1);
 }
 
-// row index of each entry by binary search of rowptr, then scatter into the
-// column's slot; the order inside a column is fixed afterwards by col_sort
-__global__ void col_scatter(const long long *__restrict__ rowptr, int nrows,
-                            const int *__restrict__ cols, const float *__restrict__ vals,
-                            long long nnz, const long long *__restrict__ colptr,
-                            int *__restrict__ cursor, int *__restrict__ trow,
-                            float *__restrict__ tval) {
+__global__ void iota_int(int *__restrict__ v, long long n) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
+    if (e < n) v[e] = (int)e;
+}
+
+// A^T gather: entry k of the column-sorted order is A's entry e = idx[k];
+// its row is found by binary search of rowptr.  The sort is stable and A's
+// entries are in row order, so rows ascend inside each column -- the
+// summation order of the reference's sequential COO loop
+// (least_squares.cpp:440, :479), which keeps SpMV^T deterministic.
+__global__ void col_gather(const long long *__restrict__ rowptr, int nrows,
+                           const float *__restrict__ vals, const int *__restrict__ idx,
+                           long long nnz, int *__restrict__ trow, float *__restrict__ tval) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const long long e = idx[k];
     int lo = 0, hi = nrows - 1;
     while (lo < hi) {  // last row with rowptr[row] <= e
         const int mid = (lo + hi + 1) >> 1;
         if (rowptr[mid] <= e) lo = mid; else hi = mid - 1;
     }
-    const int c = cols[e];
-    const long long pos = colptr[c] + atomicAdd(&cursor[c], 1);
-    trow[pos] = lo;
-    tval[pos] = vals[e];
-}
-
-// sort each column's (row, value) pairs by row: deterministic SpMV^T order
-__global__ void col_sort(const long long *__restrict__ colptr, int ncol, int *__restrict__ trow,
-                         float *__restrict__ tval) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
-    const long long b = colptr[c], n = colptr[c + 1] - b;
-    int *R = trow + b;
-    float *V = tval + b;
-    const int gaps[10] = {4001, 1750, 701, 301, 132, 57, 23, 10, 4, 1};
-    for (int gi = 0; gi < 10; gi++) {
-        const int gap = gaps[gi];
-        for (long long u = gap; u < n; u++) {
-            const int tr = R[u];
-            const float tv = V[u];
-            long long w = u;
-            while (w >= gap && R[w - gap] > tr) { R[w] = R[w - gap]; V[w] = V[w - gap]; w -= gap; }
-            R[w] = tr;
-            V[w] = tv;
-        }
-    }
+    trow[k] = lo;
+    tval[k] = vals[e];
 }
 
 // y[i] = sum_e A[i, col[e]] x[col[e]]: 16 lanes per row, fixed reduction tree
@@ -549,13 +535,28 @@ void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
     ST_REQUIRE(tot == annz, ST_ERR_HIP, "transpose count mismatch");
     ST_HIP(hipMemcpyAsync(t.t_colptr.as<long long>() + n_model, &tot, sizeof(long long),
                           hipMemcpyHostToDevice, ctx->stream));
-    ST_HIP(hipMemsetAsync(t.t_cursor.ptr, 0, sizeof(int) * (n_model + 1), ctx->stream));
-    hipLaunchKernelGGL(col_scatter, dim3(nblk256(annz)), dim3(256), 0, ctx->stream,
-                       t.rowptr.as<long long>(), (int)N, t.cols.as<int>(), t.vals.as<float>(), annz,
-                       t.t_colptr.as<long long>(), t.t_cursor.as<int>(), t.t_rows.as<int>(),
-                       t.t_vals.as<float>());
-    hipLaunchKernelGGL(col_sort, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream,
-                       t.t_colptr.as<long long>(), n_model, t.t_rows.as<int>(), t.t_vals.as<float>());
+    // stable radix sort of the entry indices by column (the bits a column
+    // index needs), then gather rows and values in that order
+    ST_REQUIRE(annz < (1LL << 31), ST_ERR_CAPACITY, "too many non-zeros for the transpose");
+    t.t_keys.reserve(sizeof(unsigned) * annz);
+    t.t_idx0.reserve(sizeof(int) * annz);
+    t.t_idx1.reserve(sizeof(int) * annz);
+    hipLaunchKernelGGL(iota_int, dim3(nblk256(annz)), dim3(256), 0, ctx->stream,
+                       t.t_idx0.as<int>(), annz);
+    unsigned bits = 1;
+    while (bits < 32 && (1u << bits) < (unsigned)n_model) bits++;
+    size_t tmp = 0;
+    const unsigned *keys_in = reinterpret_cast<const unsigned *>(t.cols.as<int>());
+    ST_HIP(rocprim::radix_sort_pairs(nullptr, tmp, keys_in, t.t_keys.as<unsigned>(),
+                                     t.t_idx0.as<int>(), t.t_idx1.as<int>(), (size_t)annz, 0u, bits,
+                                     ctx->stream));
+    t.t_sort.reserve(std::max<size_t>(tmp, 16));
+    ST_HIP(rocprim::radix_sort_pairs(t.t_sort.ptr, tmp, keys_in, t.t_keys.as<unsigned>(),
+                                     t.t_idx0.as<int>(), t.t_idx1.as<int>(), (size_t)annz, 0u, bits,
+                                     ctx->stream));
+    hipLaunchKernelGGL(col_gather, dim3(nblk256(annz)), dim3(256), 0, ctx->stream,
+                       t.rowptr.as<long long>(), (int)N, t.vals.as<float>(), t.t_idx1.as<int>(),
+                       annz, t.t_rows.as<int>(), t.t_vals.as<float>());
     ST_HIP(hipGetLastError());
 
     t.N = (int)N; t.M = n_model; t.annz = annz; t.n_model = n_model; t.order = order;
