@@ -1,6 +1,7 @@
 #!/bin/bash
-# Builds the reference's own CPU code (accurate_FSM.cpp, classic.cpp and the
-# utilities they need) from the reference checkout, with the driver
+# Builds the reference's own CPU code (accurate_FSM.cpp, classic.cpp, the
+# acoustic modelling functions of utils/wave/acoustic.cpp and the utilities
+# they need) from the reference checkout, with the driver
 # oracle/ref_driver.cpp, into oracle/_ref/libstref.so.  Test infrastructure:
 # it pins the restatement in oracle/oracle.c and can time the reference CPU
 # path.  Output goes only to oracle/_ref/; nothing is written under REF.
@@ -22,6 +23,7 @@ g++ -std=c++17 -O2 -ffp-contract=off -fPIC -shared -Wno-unknown-pragmas -w \
     "$SRC/utils/file_manager/file_manager.cpp" \
     "$SRC/utils/interpolation/trilinear.cpp" \
     "$SRC/utils/smoothing/gaussian.cpp" \
+    "$SRC/utils/wave/acoustic.cpp" \
     "$SRC/geometry/regular/regular.cpp" \
     "$SRC/geometry/circular/circular.cpp" \
     "$HERE/ref_driver.cpp" \

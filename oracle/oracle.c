@@ -1158,3 +1158,152 @@ void or_model_update(float *slowness, const float *xdm,
     memcpy(velocity, update, sizeof(float) * nPoints);
     free(update);
 }
+
+
+/* ====================================================================== */
+/* data synthesis                                                          */
+/* ====================================================================== */
+
+/* dampers (acoustic.cpp:109-143): d1[i] = exp(-(factor (nb - i))^2);
+ * d2[a + b nb] = d1[a] + d1[b] - 1; d3[i + j nb + k nb^2] = d2(i,j) +
+ * d2(j,k) + d2(i,k) before the -1 (summed in that order) - 5. */
+void or_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb)
+{
+    memset(d1, 0, sizeof(float) * nb);
+    memset(d2, 0, sizeof(float) * nb * nb);
+    memset(d3, 0, sizeof(float) * (size_t)nb * nb * nb);
+    for (int i = 0; i < nb; i++) {
+        const float a = factor * (float)(nb - i);
+        d1[i] = expf(-(a * a));
+    }
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++) {
+            d2[j + i * nb] += d1[i];
+            d2[i + j * nb] += d1[i];
+        }
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++)
+            for (int k = 0; k < nb; k++) {
+                float *c = d3 + i + j * nb + (size_t)k * nb * nb;
+                *c += d2[i + j * nb];
+                *c += d2[j + k * nb];
+                *c += d2[i + k * nb];
+            }
+    for (int q = 0; q < nb * nb; q++) d2[q] -= 1.0f;
+    for (long q = 0; q < (long)nb * nb * nb; q++) d3[q] -= 5.0f;
+}
+
+/* depth of index i into the absorbing layer of an axis of length n (0 at
+ * the outer face side ... nb-1 next to the model), -1 outside the layer */
+static int layer_index(int i, int n, int nb)
+{
+    if (i < nb) return i;
+    if (i >= n - nb) return nb - (i - (n - nb)) - 1;
+    return -1;
+}
+
+/* the damper of cell (z i, x j, y k): 1 in the model, the 1D, 2D or 3D
+ * profile where one, two or three axes are inside the layer
+ * (the case list of acoustic.cpp:183-296) */
+static float damper_at(int i, int j, int k, int nb, int nxx, int nyy, int nzz,
+                       const float *d1, const float *d2, const float *d3)
+{
+    const int lz = layer_index(i, nzz, nb), lx = layer_index(j, nxx, nb), ly = layer_index(k, nyy, nb);
+    const int n = (lz >= 0) + (lx >= 0) + (ly >= 0);
+    if (n == 0) return 1.0f;
+    if (n == 1) return d1[lz >= 0 ? lz : (lx >= 0 ? lx : ly)];
+    if (n == 2) {
+        if (lz < 0) return d2[lx + ly * nb];
+        if (lx < 0) return d2[lz + ly * nb];
+        return d2[lz + lx * nb];
+    }
+    return d3[lz + lx * nb + (size_t)ly * nb * nb];
+}
+
+void or_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float dy, float dz,
+                      int nb, float factor, const float *wavelet, int nt, float dt,
+                      float sx, float sy, float sz, const float *rx, const float *ry,
+                      const float *rz, int nrec, float *seismogram)
+{
+    const int nxx = nx + 2 * nb, nyy = ny + 2 * nb, nzz = nz + 2 * nb;
+    const long N = (long)nxx * nyy * nzz, NZ = nzz, NXZ = (long)nxx * nzz;
+    float *V = (float *)malloc(sizeof(float) * N);
+    float *P0 = (float *)calloc(N, sizeof(float));   /* U_pas */
+    float *P1 = (float *)calloc(N, sizeof(float));   /* U_pre */
+    float *P2 = (float *)calloc(N, sizeof(float));   /* U_fut */
+    float *d1 = (float *)malloc(sizeof(float) * nb), *d2 = (float *)malloc(sizeof(float) * nb * nb);
+    float *d3 = (float *)malloc(sizeof(float) * (size_t)nb * nb * nb);
+    /* expand (acoustic.cpp:16-85): edge replication = clamped index */
+    for (int k = 0; k < nyy; k++)
+        for (int j = 0; j < nxx; j++)
+            for (int i = 0; i < nzz; i++) {
+                const int zi = imin(imax(i - nb, 0), nz - 1), xi = imin(imax(j - nb, 0), nx - 1),
+                          yi = imin(imax(k - nb, 0), ny - 1);
+                V[i + j * NZ + k * NXZ] = vel[zi + (long)xi * nz + (long)yi * nx * nz];
+            }
+    or_acoustic_dampers(d1, d2, d3, factor, nb);
+    const long sId = ((int)(sz / dz) + nb) + (long)((int)(sx / dx) + nb) * NZ +
+                     (long)((int)(sy / dy) + nb) * NXZ;
+    const float cz = 5040.0f * (dz * dz), cx = 5040.0f * (dx * dx), cy = 5040.0f * (dy * dy);
+    for (int t = 0; t < nt; t++) {
+        P1[sId] += wavelet[t];                                   /* applyWavelet */
+        for (int k = 4; k < nyy - 4; k++)                        /* wavePropagation */
+            for (int j = 4; j < nxx - 4; j++)
+                for (int i = 4; i < nzz - 4; i++) {
+                    const long c = i + j * NZ + k * NXZ;
+#define ST8(s) (-9.0f * (P1[c - 4 * (s)] + P1[c + 4 * (s)]) + 128.0f * (P1[c - 3 * (s)] + P1[c + 3 * (s)]) \
+                - 1008.0f * (P1[c - 2 * (s)] + P1[c + 2 * (s)]) + 8064.0f * (P1[c - (s)] + P1[c + (s)]) \
+                - 14350.0f * P1[c])
+                    const float px = ST8(NZ) / cx;
+                    const float py = ST8(NXZ) / cy;
+                    const float pz = ST8(1) / cz;
+#undef ST8
+                    const float d = damper_at(i, j, k, nb, nxx, nyy, nzz, d1, d2, d3);
+                    const float lap = px + py + pz;
+                    P2[c] = V[c] * V[c] * (dt * dt) * lap + 2.0f * P1[c] - P0[c];
+                    P2[c] *= d;
+                    P1[c] *= d;
+                    P0[c] *= d;
+                }
+        float *tmp = P0; P0 = P1; P1 = P2; P2 = tmp;             /* wavefieldUpdate */
+        memcpy(P2, P1, sizeof(float) * N);                       /* U_fut keeps its value */
+        for (int r = 0; r < nrec; r++) {                         /* buildSeismogram */
+            const long c = ((int)(rz[r] / dz) + nb) + (long)((int)(rx[r] / dx) + nb) * NZ +
+                           (long)((int)(ry[r] / dy) + nb) * NXZ;
+            seismogram[t + (long)r * nt] = P1[c];
+        }
+    }
+    free(V); free(P0); free(P1); free(P2); free(d1); free(d2); free(d3);
+}
+
+void or_auto_pick(const float *seis, int ntraces, int nt, float dt, float window,
+                  float amp_cut, float shift, float *picks)
+{
+    const int iw = (int)(window / dt) + 1;
+    const long n = (long)ntraces * nt;
+    float mx = seis[0];
+    for (long q = 1; q < n; q++) if (seis[q] > mx) mx = seis[q];
+    const float inv = 1.0f / mx;
+    float *x = (float *)malloc(sizeof(float) * nt);
+    float *S = (float *)malloc(sizeof(float) * nt);
+    for (int tr = 0; tr < ntraces; tr++) {
+        for (int q = 0; q < nt; q++) { x[q] = seis[(long)tr * nt + q] * inv; S[q] = 0.0f; }
+        picks[tr] = 0.0f;
+        for (int time = iw; time < nt - iw; time++) {
+            float A = 0.0f, B = 0.0f;
+            for (int k = 0; k < iw; k++) {
+                A += x[time - k] + 1e-15f;
+                B += x[time + k] + 1e-15f;
+            }
+            S[time] = fabsf(B / A * (B - A));
+            S[time] *= (B + A) * (B - A) / powf((float)(iw + time), 2.0f);
+        }
+        float am = S[0];
+        for (int q = 1; q < nt; q++) if (S[q] > am) am = S[q];
+        const float ia = 1.0f / am;
+        for (int q = 0; q < nt; q++) {
+            if (S[q] * ia > amp_cut) { picks[tr] = q * dt + shift; break; }
+        }
+    }
+    free(x); free(S);
+}

@@ -135,6 +135,26 @@ void or_model_update(float *slowness, const float *xdm,
                      int smooth, int samples, float stdv,
                      float *dm, float *velocity);
 
+/* ---- data synthesis (src/utils/wave, src/utils/auto_picking) --------- */
+
+/* dampers (acoustic.cpp:109-143): Cerjan profiles d1[nb], d2[nb*nb],
+ * d3[nb*nb*nb] (zero-initialised here) */
+void or_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
+
+/* One shot of the acoustic modelling loop (wave_main.cpp:129-166 with
+ * acoustic.cpp: expand, applyWavelet, wavePropagation, wavefieldUpdate,
+ * buildSeismogram), serial; seismogram[nrec][nt]. */
+void or_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float dy, float dz,
+                      int nb, float factor, const float *wavelet, int nt, float dt,
+                      float sx, float sy, float sz, const float *rx, const float *ry,
+                      const float *rz, int nrec, float *seismogram);
+
+/* The auto-picker of auto_picking.cpp:28-82 on one file's traces
+ * (seis[ntraces][nt], normalised by its largest sample); picks[ntraces]
+ * (0 where no sample crosses amp_cut). */
+void or_auto_pick(const float *seis, int ntraces, int nt, float dt, float window,
+                  float amp_cut, float shift, float *picks);
+
 #ifdef __cplusplus
 }
 #endif

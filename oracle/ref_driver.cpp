@@ -9,6 +9,7 @@
 // order eikonal_main.cpp:24-41 does (minus Block_FIM, whose CUDA cannot be
 // built here).
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -22,6 +23,7 @@
 #include "utils/file_manager/file_manager.hpp"
 #include "utils/interpolation/trilinear.hpp"
 #include "utils/smoothing/gaussian.hpp"
+#include "utils/wave/acoustic.hpp"
 
 namespace {
 
@@ -196,6 +198,51 @@ int ref_regular(const char *param_file, const char *name, float *x, float *y, fl
     int all = g.all;
     delete[] g.x; delete[] g.y; delete[] g.z;
     return all;
+}
+
+// One shot of the acoustic modelling loop of wave_main.cpp:129-166 on caller
+// arrays, through the reference's own acoustic.cpp functions: expand (nb
+// replicated cells), dampers (Cerjan factor), then per time step
+// applyWavelet, wavePropagation, wavefieldUpdate, buildSeismogram.  Receiver
+// and source grid indices as wave_main.cpp:105-110,135-139.  seismogram is
+// [nrec][nt] (time fastest).
+int ref_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float dy, float dz,
+                      int nb, float factor, const float *wavelet, int nt, float dt, float sx,
+                      float sy, float sz, const float *rx, const float *ry, const float *rz,
+                      int nrec, float *seismogram) {
+    const int nxx = nx + 2 * nb, nyy = ny + 2 * nb, nzz = nz + 2 * nb;
+    const int nPoints = nxx * nyy * nzz;
+    float *V = expand(const_cast<float *>(vel), nx, ny, nz, nb);
+    std::vector<float> U_pas(nPoints), U_pre(nPoints), U_fut(nPoints);
+    std::vector<float> d1(nb, 0.f), d2(nb * nb, 0.f), d3(nb * nb * nb, 0.f);
+    dampers(d1.data(), d2.data(), d3.data(), factor, nb);
+    std::vector<int> rIdx(nrec), rIdy(nrec), rIdz(nrec);
+    for (int r = 0; r < nrec; r++) {
+        rIdx[r] = (int)(rx[r] / dx) + nb;
+        rIdy[r] = (int)(ry[r] / dy) + nb;
+        rIdz[r] = (int)(rz[r] / dz) + nb;
+    }
+    const int sId = ((int)(sz / dz) + nb) + ((int)(sx / dx) + nb) * nzz + ((int)(sy / dy) + nb) * nzz * nxx;
+    std::vector<float> w(wavelet, wavelet + nt);
+    setWaveField(U_pas.data(), U_pre.data(), U_fut.data(), nPoints);
+    for (int t = 0; t < nt; t++) {
+        applyWavelet(U_pre.data(), w.data(), t, sId);
+        wavePropagation(V, U_pas.data(), U_pre.data(), U_fut.data(), d1.data(), d2.data(), d3.data(),
+                        nb, nxx, nyy, nzz, dx, dy, dz, dt);
+        wavefieldUpdate(U_pas.data(), U_pre.data(), U_fut.data(), nPoints);
+        buildSeismogram(U_fut.data(), nxx, nyy, nzz, seismogram, t, nt, rIdx.data(), rIdy.data(),
+                        rIdz.data(), nrec);
+    }
+    delete[] V;
+    return 0;
+}
+
+// The reference's damping profiles (acoustic.cpp:109-143).
+void ref_dampers(float *d1, float *d2, float *d3, float factor, int nb) {
+    std::fill(d1, d1 + nb, 0.f);
+    std::fill(d2, d2 + nb * nb, 0.f);
+    std::fill(d3, d3 + nb * nb * nb, 0.f);
+    dampers(d1, d2, d3, factor, nb);
 }
 
 }  // extern "C"

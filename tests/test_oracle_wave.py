@@ -1,0 +1,79 @@
+"""Oracle for data synthesis: the acoustic modelling loop (src/utils/wave)
+and the auto-picker (src/utils/auto_picking), pinned against the
+reference's own build and closed forms."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from oracle_lib import P
+
+
+@pytest.mark.skipif(not orc.ref_available(), reason="reference build absent")
+def test_dampers_bit_exact_vs_reference():
+    for nb, factor in ((8, 0.028), (50, 0.0045)):
+        a = [np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3)]
+        b = [np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3)]
+        orc.oracle().or_acoustic_dampers(*(P(x) for x in a), factor, nb)
+        orc.ref().ref_dampers(*(P(x) for x in b), factor, nb)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.skipif(not orc.ref_available(), reason="reference build absent")
+@pytest.mark.parametrize("seed", [3, 4])
+def test_acoustic_shot_bit_exact_vs_reference(seed):
+    """Every sample of the seismogram equals the reference's own loop (its
+    in-place damping, where later cells read already damped neighbours,
+    included)."""
+    c = orc.acoustic_case(seed=seed)
+    a = orc.acoustic_shot(c)
+    b = orc.acoustic_shot(c, orc.ref())
+    assert np.abs(a).max() > 0
+    assert np.array_equal(a, b)
+
+
+def test_dampers_closed_form():
+    nb, f = 6, 0.05
+    d1, d2, d3 = (np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3))
+    orc.oracle().or_acoustic_dampers(P(d1), P(d2), P(d3), f, nb)
+    i = np.arange(nb)
+    assert np.allclose(d1, np.exp(-(f * (nb - i)) ** 2), rtol=1e-6)
+    D2 = d2.reshape(nb, nb)            # [b][a] = d1[a] + d1[b] - 1
+    assert np.allclose(D2, d1[None, :] + d1[:, None] - 1.0, atol=1e-6)
+    assert d1[-1] < 1.0 and d1[0] < d1[-1]
+
+
+def test_auto_pick_step_onset_closed_form():
+    """A step of height 1 at sample n0 (zero before): the first window
+    whose forward sum sees the step is time = n0 - iw + 1, so the pick is
+    (n0 - iw + 1) dt + shift; scaling the traces leaves it unchanged."""
+    nt, dt, window = 600, 0.001, 0.01
+    iw = int(np.float32(window) / np.float32(dt)) + 1     # 10: the quotient is 9.9999995 in float
+    n0 = np.array([100, 250, 333, 480])
+    seis = np.zeros((4, nt), np.float32)
+    for t, n in enumerate(n0):
+        seis[t, n:] = 1.0
+    picks = np.zeros(4, np.float32)
+    orc.oracle().or_auto_pick(P(seis), 4, nt, dt, window, 1e-15, 0.045, P(picks))
+    expect = (n0 - iw + 1).astype(np.float32) * np.float32(dt) + np.float32(0.045)
+    assert np.array_equal(picks, expect)
+    p2 = np.zeros(4, np.float32)
+    orc.oracle().or_auto_pick(P((seis * 7.5).astype(np.float32)), 4, nt, dt, window, 1e-15, 0.045,
+                              P(p2))
+    assert np.array_equal(p2, picks)
+
+
+def test_powf_square_is_not_always_the_product():
+    """The picker divides by powf(iw + time, 2).  The host libm's powf
+    rounds some exact ties of w * w the other way (4109^2 = 16883881 lies
+    halfway between two floats), so the device reads a table of powf
+    values made on the host instead of squaring."""
+    import ctypes
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    libm.powf.restype = ctypes.c_float
+    for w in range(0, 4097):
+        f = np.float32(w)
+        assert np.float32(libm.powf(float(f), 2.0)) == f * f, w
+    assert np.float32(libm.powf(4109.0, 2.0)) != np.float32(4109.0) * np.float32(4109.0)
