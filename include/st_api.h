@@ -215,6 +215,54 @@ int st_surrogate_predict(st_ctx *ctx, const float *src_xyz, int nshot, float *y)
 int st_nn_gemm(st_ctx *ctx, const float *A, const float *B, float *C, int M,
                int N, int K);
 
+/* ---- data synthesis --------------------------------------------------
+ * The acoustic modelling of src/utils/wave (8th-order Laplacian, Cerjan
+ * absorbing layers) and the first-break auto-picker of
+ * src/utils/auto_picking, on the device, bit-identical to the reference's
+ * serial CPU code (its in-place damping order included). */
+
+/* Absorbing-layer profiles on the host: d1[nb], d2[nb*nb], d3[nb^3].
+ * Replaces dampers (acoustic.cpp:100-137). */
+int st_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
+
+/* Velocity model vel[ny][nx][nz] (z fastest, m/s), spacings, nb absorbing
+ * cells per side (>= 4), damping factor and time step.  Replaces
+ * wave_main.cpp:10-48 (expand, dampers). */
+int st_acoustic_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz,
+                          float dx, float dy, float dz, int nb, float factor,
+                          float dt);
+
+/* Model nshot shots with one wavelet[nt] and record nrec receivers per
+ * shot.  Replaces the shot loop of wave_main.cpp:118-166 (setWaveField,
+ * then per step applyWavelet, wavePropagation, wavefieldUpdate,
+ * buildSeismogram).  Sources and receivers sit at cell (int)(p / d) + nb
+ * per axis and must lie in the model.  group = shots advanced per launch
+ * (0 = automatic, else 1, 2 or 4).  seis (nullable):
+ * [nshot][nrec][nt] (the reference's seismogram[t + r nt] per shot);
+ * ufut_last (nullable): the last shot's final U_fut (nyy*nxx*nzz padded
+ * cells, z fastest); device_ms (nullable): device time of the call. */
+int st_acoustic_shots(st_ctx *ctx, const float *wavelet, int nt,
+                      const float *src_xyz, int nshot, const float *rcv_xyz,
+                      int nrec, int group, float *seis, float *ufut_last,
+                      double *device_ms);
+
+/* As st_acoustic_shots, and pick every trace on the device without
+ * returning the seismograms: each shot's seismogram is one file of the
+ * auto-picker (normalised by its own largest sample).  picks[nshot][nrec]. */
+int st_acoustic_picks(st_ctx *ctx, const float *wavelet, int nt,
+                      const float *src_xyz, int nshot, const float *rcv_xyz,
+                      int nrec, int group, float window, float amp_cut,
+                      float shift, float *picks, double *device_ms);
+
+/* The auto-picker over one file of traces seis[ntraces][nt] (nt <= 8192):
+ * normalise by the file's largest sample, energy ratio of the iw =
+ * (int)(window / dt) + 1 samples after and before each time, first time
+ * whose ratio over its maximum exceeds amp_cut; picks[ntraces] = that time
+ * * dt + shift, 0 without a crossing.  Replaces auto_picking.cpp:26-83
+ * (shift 0.045 s and amp_cut 1e-15 there). */
+int st_auto_pick(st_ctx *ctx, const float *seis, int ntraces, int nt, float dt,
+                 float window, float amp_cut, float shift, float *picks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1164,7 +1164,7 @@ void or_model_update(float *slowness, const float *xdm,
 /* data synthesis                                                          */
 /* ====================================================================== */
 
-/* dampers (acoustic.cpp:109-143): d1[i] = exp(-(factor (nb - i))^2);
+/* dampers (acoustic.cpp:100-137): d1[i] = exp(-(factor (nb - i))^2);
  * d2[a + b nb] = d1[a] + d1[b] - 1; d3[i + j nb + k nb^2] = d2(i,j) +
  * d2(j,k) + d2(i,k) before the -1 (summed in that order) - 5. */
 void or_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb)
@@ -1204,7 +1204,7 @@ static int layer_index(int i, int n, int nb)
 
 /* the damper of cell (z i, x j, y k): 1 in the model, the 1D, 2D or 3D
  * profile where one, two or three axes are inside the layer
- * (the case list of acoustic.cpp:183-296) */
+ * (the case list of acoustic.cpp:192-302) */
 static float damper_at(int i, int j, int k, int nb, int nxx, int nyy, int nzz,
                        const float *d1, const float *d2, const float *d3)
 {
@@ -1223,7 +1223,7 @@ static float damper_at(int i, int j, int k, int nb, int nxx, int nyy, int nzz,
 void or_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float dy, float dz,
                       int nb, float factor, const float *wavelet, int nt, float dt,
                       float sx, float sy, float sz, const float *rx, const float *ry,
-                      const float *rz, int nrec, float *seismogram)
+                      const float *rz, int nrec, float *seismogram, float *ufut)
 {
     const int nxx = nx + 2 * nb, nyy = ny + 2 * nb, nzz = nz + 2 * nb;
     const long N = (long)nxx * nyy * nzz, NZ = nzz, NXZ = (long)nxx * nzz;
@@ -1273,6 +1273,7 @@ void or_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float 
             seismogram[t + (long)r * nt] = P1[c];
         }
     }
+    if (ufut) memcpy(ufut, P1, sizeof(float) * N);
     free(V); free(P0); free(P1); free(P2); free(d1); free(d2); free(d3);
 }
 
@@ -1295,8 +1296,10 @@ void or_auto_pick(const float *seis, int ntraces, int nt, float dt, float window
                 A += x[time - k] + 1e-15f;
                 B += x[time + k] + 1e-15f;
             }
+            /* powf(iw + time, 2.0f): the square, as optimised builds fold it */
+            const float w = (float)(iw + time);
             S[time] = fabsf(B / A * (B - A));
-            S[time] *= (B + A) * (B - A) / powf((float)(iw + time), 2.0f);
+            S[time] *= (B + A) * (B - A) / (w * w);
         }
         float am = S[0];
         for (int q = 1; q < nt; q++) if (S[q] > am) am = S[q];

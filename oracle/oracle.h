@@ -137,17 +137,18 @@ void or_model_update(float *slowness, const float *xdm,
 
 /* ---- data synthesis (src/utils/wave, src/utils/auto_picking) --------- */
 
-/* dampers (acoustic.cpp:109-143): Cerjan profiles d1[nb], d2[nb*nb],
+/* dampers (acoustic.cpp:100-137): Cerjan profiles d1[nb], d2[nb*nb],
  * d3[nb*nb*nb] (zero-initialised here) */
 void or_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
 
 /* One shot of the acoustic modelling loop (wave_main.cpp:129-166 with
  * acoustic.cpp: expand, applyWavelet, wavePropagation, wavefieldUpdate,
- * buildSeismogram), serial; seismogram[nrec][nt]. */
+ * buildSeismogram), serial; seismogram[nrec][nt]; ufut (nullable): the
+ * final U_fut, padded nyy x nxx x nzz, z fastest. */
 void or_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float dy, float dz,
                       int nb, float factor, const float *wavelet, int nt, float dt,
                       float sx, float sy, float sz, const float *rx, const float *ry,
-                      const float *rz, int nrec, float *seismogram);
+                      const float *rz, int nrec, float *seismogram, float *ufut);
 
 /* The auto-picker of auto_picking.cpp:28-82 on one file's traces
  * (seis[ntraces][nt], normalised by its largest sample); picks[ntraces]

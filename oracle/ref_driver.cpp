@@ -209,7 +209,7 @@ int ref_regular(const char *param_file, const char *name, float *x, float *y, fl
 int ref_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float dy, float dz,
                       int nb, float factor, const float *wavelet, int nt, float dt, float sx,
                       float sy, float sz, const float *rx, const float *ry, const float *rz,
-                      int nrec, float *seismogram) {
+                      int nrec, float *seismogram, float *ufut) {
     const int nxx = nx + 2 * nb, nyy = ny + 2 * nb, nzz = nz + 2 * nb;
     const int nPoints = nxx * nyy * nzz;
     float *V = expand(const_cast<float *>(vel), nx, ny, nz, nb);
@@ -233,11 +233,12 @@ int ref_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float 
         buildSeismogram(U_fut.data(), nxx, nyy, nzz, seismogram, t, nt, rIdx.data(), rIdy.data(),
                         rIdz.data(), nrec);
     }
+    if (ufut) std::copy(U_fut.begin(), U_fut.end(), ufut);
     delete[] V;
     return 0;
 }
 
-// The reference's damping profiles (acoustic.cpp:109-143).
+// The reference's damping profiles (acoustic.cpp:100-137).
 void ref_dampers(float *d1, float *d2, float *d3, float factor, int nb) {
     std::fill(d1, d1 + nb, 0.f);
     std::fill(d2, d2 + nb * nb, 0.f);

@@ -99,6 +99,22 @@ SIGNATURES = [
     ("st_surrogate_predict", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p]),
     ("st_nn_gemm", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int]),
+    ("st_acoustic_dampers", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_float,
+                                           ctypes.c_int]),
+    ("st_acoustic_set_model", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                             ctypes.c_float, ctypes.c_float]),
+    ("st_acoustic_shots", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                         ctypes.c_int, c_float_p, ctypes.c_int, ctypes.c_int,
+                                         c_float_p, c_float_p, ctypes.POINTER(ctypes.c_double)]),
+    ("st_acoustic_picks", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                         ctypes.c_int, c_float_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         c_float_p, ctypes.POINTER(ctypes.c_double)]),
+    ("st_auto_pick", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, c_float_p]),
 ]
 
 _lib = None

@@ -138,6 +138,59 @@ class Context:
                                              rcv.shape[0], _ptr(out)))
         return out
 
+    # ---- data synthesis (src/utils/wave, src/utils/auto_picking) ----
+    def acoustic_set_model(self, vel, spacing, nb=50, factor=0.0045, dt=0.001):
+        """Velocity model vel[ny, nx, nz] (m/s) for the acoustic modelling;
+        spacing = h or (dx, dy, dz); nb absorbing cells per side."""
+        vel = _f32(vel)
+        ny, nx, nz = vel.shape
+        dx, dy, dz = (spacing,) * 3 if np.isscalar(spacing) else spacing
+        self._check(self.lib.st_acoustic_set_model(self.h, _ptr(vel), nx, ny, nz, dx, dy, dz,
+                                                   int(nb), factor, dt))
+        self._acoustic = (nx, ny, nz, int(nb))
+
+    def acoustic_shots(self, wavelet, src, rcv, group=0, return_ufut=False):
+        """Seismograms [nshot, nrec, nt] of the acoustic modelling (and the
+        last shot's final padded U_fut [nyy, nxx, nzz]); returns (seis, ufut
+        or None, device ms)."""
+        w = _f32(wavelet, (-1,))
+        src = _f32(src, (-1, 3))
+        rcv = _f32(rcv, (-1, 3))
+        ns, nr, nt = src.shape[0], rcv.shape[0], w.shape[0]
+        seis = np.empty((ns, nr, nt), np.float32)
+        uf = None
+        if return_ufut:
+            nx, ny, nz, nb = self._acoustic
+            uf = np.empty((ny + 2 * nb, nx + 2 * nb, nz + 2 * nb), np.float32)
+        ms = ctypes.c_double()
+        self._check(self.lib.st_acoustic_shots(self.h, _ptr(w), nt, _ptr(src), ns, _ptr(rcv), nr,
+                                               int(group), _ptr(seis), _ptr(uf),
+                                               ctypes.byref(ms)))
+        return seis, uf, ms.value
+
+    def acoustic_picks(self, wavelet, src, rcv, window=0.01, amp_cut=1e-15, shift=0.045,
+                       group=0):
+        """Model and auto-pick on the device: (picks[nshot, nrec], device ms)."""
+        w = _f32(wavelet, (-1,))
+        src = _f32(src, (-1, 3))
+        rcv = _f32(rcv, (-1, 3))
+        picks = np.empty((src.shape[0], rcv.shape[0]), np.float32)
+        ms = ctypes.c_double()
+        self._check(self.lib.st_acoustic_picks(self.h, _ptr(w), w.shape[0], _ptr(src),
+                                               src.shape[0], _ptr(rcv), rcv.shape[0], int(group),
+                                               window, amp_cut, shift, _ptr(picks),
+                                               ctypes.byref(ms)))
+        return picks, ms.value
+
+    def auto_pick(self, seis, dt, window=0.01, amp_cut=1e-15, shift=0.045):
+        """The auto-picker over one file seis[ntraces, nt]."""
+        seis = _f32(seis)
+        ntr, nt = seis.shape
+        picks = np.empty(ntr, np.float32)
+        self._check(self.lib.st_auto_pick(self.h, _ptr(seis), ntr, nt, dt, window, amp_cut,
+                                          shift, _ptr(picks)))
+        return picks
+
     # ---- tomography ----
     def tomo_set_grid(self, dxt, dyt=None, dzt=None, max_steps=0):
         dyt = dxt if dyt is None else dyt
@@ -285,3 +338,16 @@ class Context:
         C = np.empty((M, N), np.float32)
         self._check(self.lib.st_nn_gemm(self.h, _ptr(A), _ptr(B), _ptr(C), M, N, K))
         return C
+
+
+def acoustic_dampers(factor, nb):
+    """Absorbing-layer profiles (d1[nb], d2[nb, nb], d3[nb, nb, nb]) made on
+    the host (no device needed)."""
+    lib = _lib.load()
+    d1 = np.zeros(nb, np.float32)
+    d2 = np.zeros((nb, nb), np.float32)
+    d3 = np.zeros((nb, nb, nb), np.float32)
+    rc = lib.st_acoustic_dampers(_ptr(d1), _ptr(d2), _ptr(d3), factor, int(nb))
+    if rc != 0:
+        raise ValueError(f"st_acoustic_dampers: status {rc}")
+    return d1, d2, d3

@@ -30,6 +30,15 @@ bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float
 float surrogate_cold_radius(st_ctx *ctx);
 void tomo_release(st_ctx *ctx);
 void surrogate_release(st_ctx *ctx);
+void wave_release(st_ctx *ctx);
+void acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
+void wave_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz, float dx, float dy,
+                    float dz, int nb, float factor, float dt);
+void wave_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz, int nshot,
+                const float *rcv_xyz, int nrec, int group, float *seis_out, float *ufut_out,
+                const float *pick_params, float *picks_out, double *device_ms);
+void wave_pick(st_ctx *ctx, const float *seis, int ntraces, int nt, float dt, float window,
+               float amp_cut, float shift, float *picks);
 struct TomoState;
 TomoState &tomo(st_ctx *ctx);
 bool tomo_grid(st_ctx *ctx, int *n, float *d, int *max_steps, int *max_runs);
@@ -132,6 +141,7 @@ int st_ctx_destroy(st_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     tomo_release(ctx);
     surrogate_release(ctx);
+    wave_release(ctx);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (auto *b : {&ctx->slow_zxy, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
@@ -675,6 +685,57 @@ int st_model_update(st_ctx *ctx, const float *xdm, int smooth, int samples, floa
             ST_REQUIRE(dx, ST_ERR_STATE, "no solution: call st_optimize or pass xdm");
         }
         tomo_model_update(ctx, dx, smooth, samples, stdv, dm, velocity);
+    });
+}
+
+
+// ---- data synthesis ----------------------------------------------------
+
+int st_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb) {
+    if (!d1 || !d2 || !d3 || nb <= 0) return ST_ERR_ARG;
+    acoustic_dampers(d1, d2, d3, factor, nb);
+    return ST_OK;
+}
+
+int st_acoustic_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz, float dx,
+                          float dy, float dz, int nb, float factor, float dt) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        wave_set_model(ctx, vel, nx, ny, nz, dx, dy, dz, nb, factor, dt);
+    });
+}
+
+int st_acoustic_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz, int nshot,
+                      const float *rcv_xyz, int nrec, int group, float *seis, float *ufut_last,
+                      double *device_ms) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        wave_shots(ctx, wavelet, nt, src_xyz, nshot, rcv_xyz, nrec, group, seis, ufut_last,
+                   nullptr, nullptr, device_ms);
+    });
+}
+
+int st_acoustic_picks(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz, int nshot,
+                      const float *rcv_xyz, int nrec, int group, float window, float amp_cut,
+                      float shift, float *picks, double *device_ms) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(picks, ST_ERR_ARG, "picks is null");
+        set_device(ctx);
+        const float pp[3] = {window, amp_cut, shift};
+        wave_shots(ctx, wavelet, nt, src_xyz, nshot, rcv_xyz, nrec, group, nullptr, nullptr, pp,
+                   picks, device_ms);
+    });
+}
+
+int st_auto_pick(st_ctx *ctx, const float *seis, int ntraces, int nt, float dt, float window,
+                 float amp_cut, float shift, float *picks) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        wave_pick(ctx, seis, ntraces, nt, dt, window, amp_cut, shift, picks);
     });
 }
 

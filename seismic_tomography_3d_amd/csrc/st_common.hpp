@@ -139,7 +139,8 @@ struct st_ctx {
     hipGraph_t graph = nullptr;
     long long graph_key = -1;
 
-    // opaque extension state (tomography, surrogate)
+    // opaque extension state (tomography, surrogate, acoustic modelling)
     void *tomo = nullptr;
     void *surr = nullptr;
+    void *wave = nullptr;
 };

@@ -65,7 +65,7 @@ def oracle():
             "or_model_update": (None, [fp, fp, I, I, I, F, F, F, I, I, I, F, F, F, I, I, F, fp, fp]),
             "or_acoustic_dampers": (None, [fp, fp, fp, F, I]),
             "or_acoustic_shot": (None, [fp, I, I, I, F, F, F, I, F, fp, I, F, F, F, F,
-                                        fp, fp, fp, I, fp]),
+                                        fp, fp, fp, I, fp, fp]),
             "or_auto_pick": (None, [fp, I, I, F, F, F, F, fp]),
         }
         for name, (res, args) in sig.items():
@@ -100,7 +100,7 @@ def ref():
         r.ref_regular.argtypes = [ctypes.c_char_p, ctypes.c_char_p, fp, fp, fp, I]
         r.ref_regular.restype = I
         r.ref_acoustic_shot.argtypes = [fp, I, I, I, F, F, F, I, F, fp, I, F, F, F, F,
-                                        fp, fp, fp, I, fp]
+                                        fp, fp, fp, I, fp, fp]
         r.ref_acoustic_shot.restype = I
         r.ref_dampers.argtypes = [fp, fp, fp, F, I]
         r.ref_dampers.restype = None
@@ -176,30 +176,38 @@ def fsm(vel, spacing, src):
     return T
 
 
-def acoustic_case(nx=22, ny=18, nz=16, h=10.0, nb=8, nt=120, dt=0.001, seed=3):
-    """A small layered model, a Ricker-like wavelet, one source, receivers."""
+def acoustic_case(nx=22, ny=18, nz=16, h=10.0, nb=8, nt=120, dt=0.001, seed=3, nrec=9,
+                  spacing=None):
+    """A small layered model with +-3 % noise, a 40 Hz Ricker wavelet
+    delayed 20 ms, one source and nrec random receivers (inside the model).
+    spacing = (dx, dy, dz) overrides h."""
     rng = np.random.default_rng(seed)
+    dx, dy, dz = spacing if spacing is not None else (h, h, h)
     vel = np.empty((ny, nx, nz), np.float32)
     vel[:] = np.linspace(1500.0, 2600.0, nz, dtype=np.float32)[None, None, :]
     vel *= rng.uniform(0.97, 1.03, vel.shape).astype(np.float32)
     t = np.arange(nt, dtype=np.float32) * dt - 0.02
     f = 40.0
     w = ((1 - 2 * (np.pi * f * t) ** 2) * np.exp(-(np.pi * f * t) ** 2)).astype(np.float32)
-    src = np.array([nx * h * 0.45, ny * h * 0.55, nz * h * 0.3], np.float32)
-    rr = rng.uniform(0, 1, (9, 3)).astype(np.float32)
-    rcv = (rr * np.array([(nx - 1) * h, (ny - 1) * h, (nz - 1) * h], np.float32)).astype(np.float32)
-    return dict(vel=vel, nx=nx, ny=ny, nz=nz, h=h, nb=nb, nt=nt, dt=dt, wavelet=w, src=src,
-                rcv=rcv, factor=0.0045 * 50 / nb)
+    ext = np.array([(nx - 1) * dx, (ny - 1) * dy, (nz - 1) * dz], np.float32)
+    src = (np.array([0.45, 0.55, 0.3], np.float32) * ext).astype(np.float32)
+    rcv = (rng.uniform(0, 1, (nrec, 3)).astype(np.float32) * ext).astype(np.float32)
+    return dict(vel=vel, nx=nx, ny=ny, nz=nz, d=(dx, dy, dz), nb=nb, nt=nt, dt=dt, wavelet=w,
+                src=src, rcv=rcv, factor=0.0045 * 50 / nb)
 
 
-def acoustic_shot(c, lib=None):
+def acoustic_shot(c, lib=None, src=None, want_ufut=False):
+    """Seismogram [nrec, nt] (and the final padded U_fut) of one shot."""
     lib = lib or oracle()
     rcv = c["rcv"]
+    src = c["src"] if src is None else src
     rx, ry, rz = (np.ascontiguousarray(rcv[:, i]) for i in range(3))
     seis = np.zeros((rcv.shape[0], c["nt"]), np.float32)
+    nb = c["nb"]
+    uf = np.zeros((c["ny"] + 2 * nb, c["nx"] + 2 * nb, c["nz"] + 2 * nb), np.float32) \
+        if want_ufut else None
     fn = lib.or_acoustic_shot if hasattr(lib, "or_acoustic_shot") else lib.ref_acoustic_shot
-    h = c["h"]
-    fn(P(np.ascontiguousarray(c["vel"])), c["nx"], c["ny"], c["nz"], h, h, h, c["nb"], c["factor"],
-       P(c["wavelet"]), c["nt"], c["dt"], *(float(v) for v in c["src"]), P(rx), P(ry), P(rz),
-       rcv.shape[0], P(seis))
-    return seis
+    fn(P(np.ascontiguousarray(c["vel"])), c["nx"], c["ny"], c["nz"], *c["d"], nb, c["factor"],
+       P(c["wavelet"]), c["nt"], c["dt"], *(float(v) for v in src), P(rx), P(ry), P(rz),
+       rcv.shape[0], P(seis), P(uf) if uf is not None else None)
+    return (seis, uf) if want_ufut else seis

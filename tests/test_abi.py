@@ -37,3 +37,23 @@ def test_library_exports_every_declared_symbol():
 def test_python_binding_covers_header():
     from seismic_tomography_3d_amd import _lib
     assert set(declared()) <= set(_lib.exported_names())
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libst_hip.so not built")
+@pytest.mark.parametrize("nb,factor", [(8, 0.028), (50, 0.0045), (4, 0.1)])
+def test_acoustic_dampers_host_entry_matches_oracle(nb, factor):
+    """st_acoustic_dampers runs on the host only: its tables equal the
+    oracle's (and the reference build's, where present) bit for bit."""
+    import numpy as np
+    import oracle_lib as orc
+    from oracle_lib import P
+    from seismic_tomography_3d_amd.api import acoustic_dampers
+    d1, d2, d3 = acoustic_dampers(factor, nb)
+    o = [np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3)]
+    orc.oracle().or_acoustic_dampers(*(P(x) for x in o), factor, nb)
+    assert np.array_equal(d1, o[0]) and np.array_equal(d2.ravel(), o[1])
+    assert np.array_equal(d3.ravel(), o[2])
+    if orc.ref_available():
+        r = [np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3)]
+        orc.ref().ref_dampers(*(P(x) for x in r), factor, nb)
+        assert np.array_equal(d3.ravel(), r[2])

@@ -20,16 +20,21 @@ def test_dampers_bit_exact_vs_reference():
 
 
 @pytest.mark.skipif(not orc.ref_available(), reason="reference build absent")
-@pytest.mark.parametrize("seed", [3, 4])
-def test_acoustic_shot_bit_exact_vs_reference(seed):
-    """Every sample of the seismogram equals the reference's own loop (its
-    in-place damping, where later cells read already damped neighbours,
-    included)."""
-    c = orc.acoustic_case(seed=seed)
-    a = orc.acoustic_shot(c)
-    b = orc.acoustic_shot(c, orc.ref())
+@pytest.mark.parametrize("case", [dict(seed=3), dict(seed=4, nb=5, spacing=(10.0, 12.5, 7.5)),
+                                  dict(seed=5, nb=4, nt=60)])
+def test_acoustic_shot_bit_exact_vs_reference(case):
+    """Every sample of the seismogram and every cell of the final U_fut
+    equal the reference's own loop (its in-place damping, where later cells
+    read already damped neighbours, included)."""
+    c = orc.acoustic_case(**case)
+    a, ua = orc.acoustic_shot(c, want_ufut=True)
+    b, ub = orc.acoustic_shot(c, orc.ref(), want_ufut=True)
     assert np.abs(a).max() > 0
     assert np.array_equal(a, b)
+    assert np.array_equal(ua, ub)
+    nb = c["nb"]
+    assert np.all(ua[:4] == 0) and np.all(ua[-4:] == 0)          # the shell is never written
+    assert np.count_nonzero(ua[nb:-nb, nb:-nb, nb:-nb]) > 0
 
 
 def test_dampers_closed_form():
@@ -63,11 +68,13 @@ def test_auto_pick_step_onset_closed_form():
     assert np.array_equal(p2, picks)
 
 
-def test_powf_square_is_not_always_the_product():
-    """The picker divides by powf(iw + time, 2).  The host libm's powf
-    rounds some exact ties of w * w the other way (4109^2 = 16883881 lies
-    halfway between two floats), so the device reads a table of powf
-    values made on the host instead of squaring."""
+def test_powf_square_folds_to_the_product():
+    """The reference writes powf(x, 2.0f) (V^2, dt^2, d^2, the picker's
+    (iw + t)^2).  Optimised builds fold it to x * x, while the libm call of
+    an unoptimised build rounds some exact ties the other way (4109^2 =
+    16883881 lies halfway between two floats): the restatement and the
+    device code square, as the reference's optimised build (O2, the build
+    the seismogram tests compare against) does."""
     import ctypes
     import ctypes.util
     libm = ctypes.CDLL(ctypes.util.find_library("m"))
@@ -77,3 +84,11 @@ def test_powf_square_is_not_always_the_product():
         f = np.float32(w)
         assert np.float32(libm.powf(float(f), 2.0)) == f * f, w
     assert np.float32(libm.powf(4109.0, 2.0)) != np.float32(4109.0) * np.float32(4109.0)
+    # the picker restatement squares: a step onset late in a long trace,
+    # where (iw + t)^2 passes 2^24, still picks at the closed-form sample
+    nt, dt = 6000, np.float32(0.001)
+    seis = np.zeros((1, nt), np.float32)
+    seis[0, 5000:] = 1.0
+    picks = np.zeros(1, np.float32)
+    orc.oracle().or_auto_pick(P(seis), 1, nt, float(dt), 0.01, 1e-15, 0.0, P(picks))
+    assert picks[0] == np.float32(5000 - 10 + 1) * dt
