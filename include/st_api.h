@@ -236,8 +236,9 @@ int st_acoustic_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz,
  * shot.  Replaces the shot loop of wave_main.cpp:118-166 (setWaveField,
  * then per step applyWavelet, wavePropagation, wavefieldUpdate,
  * buildSeismogram).  Sources and receivers sit at cell (int)(p / d) + nb
- * per axis and must lie in the model.  group = shots advanced per launch
- * (0 = automatic, else 1, 2 or 4).  seis (nullable):
+ * per axis and must lie in the model.  group = shots advanced together
+ * (0 = automatic: enough to fill the device; each holds three wavefields).
+ * seis (nullable):
  * [nshot][nrec][nt] (the reference's seismogram[t + r nt] per shot);
  * ufut_last (nullable): the last shot's final U_fut (nyy*nxx*nzz padded
  * cells, z fastest); device_ms (nullable): device time of the call. */

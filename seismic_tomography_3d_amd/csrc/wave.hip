@@ -24,14 +24,20 @@
 //   some exact ties the other way), and is squared here.  expf is a true
 //   libm call: the dampers are made on the host.
 //
-// Layout in HBM: padded volumes nzz x nxx x nyy (z fastest, the reference's
-// index i + j nzz + k nxx nzz); V^2 dt^2 and the damper as two such volumes
-// shared by all shots; per shot three rotating wavefields.  A step reads
-// U_pre, U_pas, V^2 dt^2 and the damper once per cell and writes U_fut
-// once: 20 bytes per cell and step for one shot; a launch that advances G
-// shots reads the two model volumes once, 8 / G + 12 bytes per shot-cell.
+// Layout in HBM: padded volumes nzz x nxx x nyy in the reference's order
+// (z fastest, then x, then y; acoustic.cpp:166-168) with the z stride
+// rounded up to nzp, a multiple of 32, so that rows start on 128-byte lines
+// and every tile row segment is whole lines; the cells z >= nzz of a row are
+// padding (zero, never read).
+// V^2 dt^2 and the damper are two such volumes shared by all shots; per
+// shot three rotating wavefields.  A cell's damper is formed from the 1D
+// profile (nb floats) in the reference's accumulation order, not stored.  A
+// step reads U_pre, U_pas and V^2 dt^2 once per cell and writes U_fut once:
+// 16 bytes per cell and step.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
+#include <utility>
 
 #include "st_common.hpp"
 
@@ -39,7 +45,6 @@ namespace st {
 
 namespace {
 
-constexpr int kWz = 64;          // tile extent along z (one wave)
 constexpr int kWaves = 4;        // waves per workgroup, along x
 
 // -------------------------------------------------------------------------
@@ -49,12 +54,16 @@ constexpr int kWaves = 4;        // waves per workgroup, along x
 // of the model outwards, i.e. reads the model at the clamped index; what is
 // stored is the coefficient V^2 dt^2 of the update (acoustic.cpp:306)
 __global__ void wave_expand(const float *__restrict__ m, float *__restrict__ out, int nx, int ny,
-                            int nz, int nb, int nxx, int nyy, int nzz, float dt2) {
-    const long long N = (long long)nxx * nyy * nzz;
+                            int nz, int nb, int nxx, int nyy, int nzz, int nzp, float dt2) {
+    const long long N = (long long)nxx * nyy * nzp;
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < N;
          q += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(q % nzz);
-        const long long r = q / nzz;
+        const int i = (int)(q % nzp);
+        const long long r = q / nzp;
+        if (i >= nzz) {
+            out[q] = 0.0f;
+            continue;
+        }
         const int j = (int)(r % nxx), k = (int)(r / nxx);
         const int zi = min(max(i - nb, 0), nz - 1), xi = min(max(j - nb, 0), nx - 1),
                   yi = min(max(k - nb, 0), ny - 1);
@@ -71,30 +80,20 @@ __device__ __forceinline__ int layer_index(int i, int n, int nb) {
     return -1;
 }
 
-// damper volume: 1 in the model, the 1D profile where one axis lies in the
-// layer, the 2D one where two do, the 3D one in the corners -- the case
-// list of acoustic.cpp:192-302, one branch per number of layered axes
-__global__ void wave_damper_volume(const float *__restrict__ d1, const float *__restrict__ d2,
-                                   const float *__restrict__ d3, float *__restrict__ out, int nb,
-                                   int nxx, int nyy, int nzz) {
-    const long long N = (long long)nxx * nyy * nzz;
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < N;
-         q += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(q % nzz);
-        const long long r = q / nzz;
-        const int j = (int)(r % nxx), k = (int)(r / nxx);
-        const int lz = layer_index(i, nzz, nb), lx = layer_index(j, nxx, nb),
-                  ly = layer_index(k, nyy, nb);
-        const int n = (lz >= 0) + (lx >= 0) + (ly >= 0);
-        float d = 1.0f;
-        if (n == 1)
-            d = d1[lz >= 0 ? lz : (lx >= 0 ? lx : ly)];
-        else if (n == 2)
-            d = lz < 0 ? d2[lx + ly * nb] : (lx < 0 ? d2[lz + ly * nb] : d2[lz + lx * nb]);
-        else if (n == 3)
-            d = d3[lz + lx * nb + (long long)ly * nb * nb];
-        out[q] = d;
-    }
+// A cell's damper from the 1D profile: 1 in the model, d1[l] where one axis
+// lies in the layer, (d1[a] + d1[b]) - 1 where two do, and in the corners
+// ((d1z + d1x) + (d1x + d1y)) + (d1z + d1y) - 5 -- the 2D and 3D tables of
+// acoustic.cpp:108-136 entry for entry (each 2D entry is one sum of two
+// profile values, each 3D entry three 2D sums added in that order).  v* is
+// the profile value of an axis, i* whether it lies in the layer.
+__device__ __forceinline__ float damper_of(float vz, bool iz, float vx, bool ix, float vy,
+                                           bool iy) {
+    const float szx = vz + vx, sxy = vx + vy, szy = vz + vy;
+    const float d3 = ((szx + sxy) + szy) - 5.0f;
+    const float d2 = (iz ? (ix ? szx : szy) : sxy) - 1.0f;
+    const float d1v = iz ? vz : (ix ? vx : vy);
+    const int n = (int)iz + (int)ix + (int)iy;
+    return n == 0 ? 1.0f : (n == 1 ? d1v : (n == 2 ? d2 : d3));
 }
 
 // -------------------------------------------------------------------------
@@ -102,175 +101,263 @@ __global__ void wave_damper_volume(const float *__restrict__ d1, const float *__
 
 struct StepArgs {
     const float *cv;       // V^2 dt^2, padded volume
-    const float *damp;     // damper, padded volume
+    const float *d1;       // 1D damping profile (nb)
     const float *prev;     // undamped U_pre of step t-1, shot stride N
     const float *cur;      // U_pre of step t (without the wavelet)
     float *out;            // U_fut of step t
-    const long long *sid;  // source cell per shot (-1: none)
+    const long long *sid;  // source cell per shot
     long long N;           // cells per padded volume
-    int nzz, nxx, nyy;
+    int nzz, nzp, nzr, nxx, nyy;   // nzr: nzz rounded up to 4 (the last quad)
+    int nb;
     int ntz, ntx;          // tiles along z and x
-    int ychunk;            // planes per workgroup
+    int nchunks, ychunk;   // chunks of y planes and their length
     float wcur, wprev;     // wavelet[t], wavelet[t-1] (0 at t = 0)
     float den_x, den_y, den_z;   // 5040 d^2
 };
 
-// One workgroup advances a 64 (z) x 4 RX (x) tile of G shots through
-// `ychunk` planes of y, marching a register queue of the y-neighbours
-// (four damped behind, four ahead) and staging the current plane with its
-// z/x halo in LDS twice: undamped (read by the neighbours at higher index)
-// and damped (read by those at lower index).
-template <int RX, int G>
-__global__ __launch_bounds__(256) void wave_step(StepArgs a) {
-    constexpr int TX = kWaves * RX, HZ = kWz + 8, HX = TX + 8;
-    __shared__ float tb[G][HX][HZ];
-    __shared__ float td[G][HX][HZ];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int bz = blockIdx.x % a.ntz, bx = blockIdx.x / a.ntz;
-    const int z0 = 4 + bz * kWz, x0 = 4 + bx * TX;
-    const int kb = 4 + blockIdx.y * a.ychunk;
-    const int ke = min(kb + a.ychunk, a.nyy - 4);
-    const long long NXZ = (long long)a.nzz * a.nxx;
-    const long long gofs = (long long)blockIdx.z * G * a.N;
-    const float *cur = a.cur + gofs;
-    const float *prev = a.prev + gofs;
-    float *out = a.out + gofs;
-    long long sid[G];
-#pragma unroll
-    for (int g = 0; g < G; g++) sid[g] = a.sid[blockIdx.z * G + g];
+typedef float f4 __attribute__((ext_vector_type(4)));
 
-    const int z = z0 + lane;
-    const bool zin = z < a.nzz, zc = z < a.nzz - 4;
-    long long col[RX];
-    bool in[RX], comp[RX];
+// Buffer resource over the cells [first, N) of one volume (raw buffer,
+// 32-bit byte offsets; a load past the end returns 0 instead of faulting)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vol_rsrc(const float *base, long long first,
+                                                           long long N) {
+    const long long bytes = (N - first) * 4;
+    const unsigned num = bytes > 0xffffffffLL ? 0xffffffffu : (unsigned)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base + first), (short)0, (int)num,
+                                             0x00020000);
+}
+__device__ __forceinline__ f4 bld4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void bst1(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst4(__amdgpu_buffer_rsrc_t r, unsigned off, f4 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(
+        __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, (int)off, 0, 0);
+}
+
+template <class F, int... J>
+__device__ __forceinline__ void unrolled(F &&f, std::integer_sequence<int, J...>) {
+    (f(std::integral_constant<int, J>{}), ...);
+}
+
+// One workgroup advances a tile of 4 LZ (z) x TX (x) cells of one shot
+// through a chunk of y planes.  A thread holds four consecutive z cells of
+// one x row (one 16-byte load per field); a wave covers 64 / LZ rows.  The
+// y-neighbours live in a ring of ten register planes (four damped behind,
+// the centre, four ahead and the one being loaded), indexed at compile time
+// by unrolling the plane loop ten times.  The current plane is staged with
+// its z/x halo in LDS twice -- undamped (read by the neighbours at higher
+// index) and damped (read by those at lower index) -- double buffered, one
+// barrier per plane; the loads of plane k + 1 are issued before the stencil
+// of plane k.  A thread's own cells come from registers: its z-neighbours
+// take two 16-byte LDS reads, its x-neighbours eight.  Workgroups are
+// numbered so that each XCD (workgroup id mod 8) receives a contiguous run
+// of tiles: x-neighbours, which share halo rows, run on the same L2.
+template <int LZ>
+__global__ __launch_bounds__(256) void wave_step(StepArgs a) {
+    constexpr int RW = 64 / LZ;            // x rows per wave
+    constexpr int TX = kWaves * RW;        // x rows per tile
+    constexpr int TZ = 4 * LZ;             // z cells per tile
+    constexpr int HX = TX + 8, HZ4 = LZ + 2;
+    constexpr int NXH = 8 * LZ;            // x-halo quads (128 or 64)
+    __shared__ f4 tb[2][HX][HZ4];
+    __shared__ f4 td[2][HX][HZ4];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int total = gridDim.x, pblk = blockIdx.x, full = total & ~7;
+    const int logical = pblk < full ? (pblk & 7) * (full >> 3) + (pblk >> 3) : pblk;
+    const int ntiles = a.ntz * a.ntx, per_shot = ntiles * a.nchunks;
+    const int shot = logical / per_shot, rem = logical - shot * per_shot;
+    const int chunk = rem / ntiles, tile = rem - chunk * ntiles;
+    const int bz = tile % a.ntz, bx = tile / a.ntz;
+    const int z0 = bz * TZ, x0 = 4 + bx * TX;
+    const int kb = 4 + chunk * a.ychunk, ke = min(kb + a.ychunk, a.nyy - 4);
+    const int nzp = a.nzp;
+    const long long NXZ = (long long)nzp * a.nxx, N = a.N;
+    const long long so = (long long)shot * N;
+    const float *cur = a.cur + so, *prev = a.prev + so;
+    float *out = a.out + so;
+    // the source: plane ks, byte offset cs in the plane, cell sj of its quad
+    const long long sq = a.sid[shot];
+    const int ks = (int)(sq / NXZ);
+    const unsigned sc = (unsigned)(sq - (long long)ks * NXZ);
+    const unsigned cs = (sc & ~3u) * 4u;
+    const int sj = (int)(sc & 3u);
+
+    // Every load reads a 16-byte quad inside the array (coordinates
+    // clamped); a value outside it lands in LDS or the ring but is never
+    // read by a computed cell, whose neighbours all lie inside.
+    const int zq = lane % LZ, row = w * RW + lane / LZ;
+    const int z = z0 + 4 * zq, x = x0 + row;
+    const unsigned col = (unsigned)(min(z, a.nzr - 4) + min(x, a.nxx - 1) * nzp) * 4u;
+    // quads are aligned and the interior starts at z = 4: a quad is either
+    // below it or starts inside it (and may end past it)
+    bool comp[4];
 #pragma unroll
-    for (int r = 0; r < RX; r++) {
-        const int x = x0 + w * RX + r;
-        in[r] = zin && x < a.nxx;
-        comp[r] = zc && x < a.nxx - 4;
-        col[r] = z + (long long)x * a.nzz;
+    for (int j = 0; j < 4; j++) comp[j] = z >= 4 && z + j < a.nzz - 4 && x < a.nxx - 4;
+    const bool any = comp[0];
+    const bool all4 = comp[3];
+    // halo quads: x halo (4 rows before the tile, damped; 4 after) by waves
+    // 0 and 1 (LZ = 16) or wave 0 (LZ = 8); z halo (one quad below each row,
+    // damped; one above) by wave 2
+    const bool xh_on = tid < NXH;
+    const int xhr = tid / LZ, xhz = tid % LZ;          // halo row 0..7
+    const bool xleft = xhr < 4;
+    const int xhx = xleft ? x0 - 4 + xhr : min(x0 + TX + xhr - 4, a.nxx - 1);
+    const unsigned oxh = (unsigned)(min(z0 + 4 * xhz, a.nzr - 4) + xhx * nzp) * 4u;
+    const int xlrow = xleft ? xhr : TX + xhr;
+    const bool zh_on = w == 2 && lane < 2 * TX;
+    const bool zbelow = lane < TX;
+    const int zhrow = zbelow ? lane : lane - TX;
+    const int zhz = zbelow ? max(z0 - 4, 0) : min(z0 + TZ, a.nzr - 4);
+    const unsigned ozh = (unsigned)(zhz + min(x0 + min(zhrow, TX - 1), a.nxx - 1) * nzp) * 4u;
+    const int zhcol = zbelow ? 0 : LZ + 1;
+
+    // profile values of the fixed axes: own row and quad, the x-halo row
+    // and quad, the z-halo quad below
+    const int nb = a.nb;
+    auto lidx = [&](int i, int n) { return i < nb ? i : (i >= n - nb ? n - 1 - i : -1); };
+    auto prof = [&](int l) { return l >= 0 ? a.d1[l] : 0.0f; };
+    const int lxo = lidx(min(x, a.nxx - 1), a.nxx);
+    const float vxo = prof(lxo);
+    float vzo[4], vzh[4], vzb[4];
+    int lzo[4], lzh[4], lzb[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        lzo[j] = lidx(min(z, a.nzr - 4) + j, a.nzz);
+        vzo[j] = prof(lzo[j]);
+        lzh[j] = lidx(min(z0 + 4 * xhz, a.nzr - 4) + j, a.nzz);
+        vzh[j] = prof(lzh[j]);
+        lzb[j] = lidx(max(z0 - 4, 0) + j, a.nzz);
+        vzb[j] = prof(lzb[j]);
     }
-    auto ld = [&](int g, long long q) {   // U_pre with the wavelet at the source
-        const float v = cur[g * a.N + q];
-        return q == sid[g] ? v + a.wcur : v;
+    const int lxh = lidx(xhx, a.nxx), lxb = lidx(min(x0 + min(zhrow, TX - 1), a.nxx - 1), a.nxx);
+    const float vxh = prof(lxh), vxb = prof(lxb);
+    // dampers of a quad in plane k
+    auto quad_damper = [&](const float *vz, const int *lz, float vx, int lx, int k) {
+        const int ly = lidx(k, a.nyy);
+        const float vy = prof(ly);
+        f4 d;
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = damper_of(vz[j], lz[j] >= 0, vx, lx >= 0, vy, ly >= 0);
+        return d;
     };
 
-    float qd[G][RX][4], qc[G][RX], qf[G][RX][4];
+    struct Pf {
+        f4 c, p, xh, zh;
+    };
+    Pf pf[2];
+    auto fetch = [&](Pf &f, int k) {
+        const long long first = (long long)k * NXZ;
+        const auto Rb = vol_rsrc(cur, first, N);
+        f.c = bld4(vol_rsrc(a.cv, first, N), col);
+        f.p = bld4(vol_rsrc(prev, first, N), col);
+        if (xh_on) f.xh = bld4(Rb, oxh);
+        if (zh_on) f.zh = bld4(Rb, ozh);
+    };
+    // add the wavelet to the source cell of a quad of U_pre of plane k
+    auto inject = [&](f4 v, int k, unsigned off, float wv) {
+        if (k == ks && off == cs) v[sj] = v[sj] + wv;
+        return v;
+    };
+
+    f4 q[10];   // ring of y planes
 #pragma unroll
-    for (int r = 0; r < RX; r++) {
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const long long qb = (long long)(kb - 4 + m) * NXZ + col[r];
-            const long long qa = (long long)(kb + 1 + m) * NXZ + col[r];
-            const float d = in[r] ? a.damp[qb] : 0.0f;
-#pragma unroll
-            for (int g = 0; g < G; g++) {
-                qd[g][r][m] = in[r] ? ld(g, qb) * d : 0.0f;
-                qf[g][r][m] = in[r] ? ld(g, qa) : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < G; g++) qc[g][r] = in[r] ? ld(g, (long long)kb * NXZ + col[r]) : 0.0f;
+    for (int m = 0; m < 9; m++) {
+        const int k = kb - 4 + m;
+        const long long first = (long long)k * NXZ;
+        f4 v = inject(bld4(vol_rsrc(cur, first, N), col), k, col, a.wcur);
+        q[m] = m < 4 ? v * quad_damper(vzo, lzo, vxo, lxo, k) : v;
     }
+    fetch(pf[0], kb);
 
-    for (int k = kb; k < ke; k++) {
-        const long long pk = (long long)k * NXZ;
-        float dk[RX];
-#pragma unroll
-        for (int r = 0; r < RX; r++) dk[r] = in[r] ? a.damp[pk + col[r]] : 0.0f;
-        __syncthreads();   // the previous plane's stencil reads are done
-#pragma unroll
-        for (int g = 0; g < G; g++)
-#pragma unroll
-            for (int r = 0; r < RX; r++) {
-                tb[g][w * RX + r + 4][lane + 4] = qc[g][r];
-                td[g][w * RX + r + 4][lane + 4] = qc[g][r] * dk[r];
+    for (int kbase = kb; kbase < ke; kbase += 10) {
+        unrolled([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            constexpr int b = J & 1, C = (J + 4) % 10, NEW = (J + 9) % 10;
+            const int k = kbase + J;
+            if (k >= ke) return;
+            Pf &cu = pf[b];
+            if (k == ks) {   // uniform: the source plane
+                if (xh_on) cu.xh = inject(cu.xh, k, oxh, a.wcur);
+                if (zh_on) cu.zh = inject(cu.zh, k, ozh, a.wcur);
+                cu.p = inject(cu.p, k, col, a.wprev);
             }
-        // x halo: the 4 rows before the tile (lower index: damped) and the 4
-        // after it (undamped), 64 lanes each
-        {
-            const int xl = x0 - 4 + w, xr = x0 + TX + w;
-            if (zin) {
-                const long long ql = pk + z + (long long)xl * a.nzz;
-                const float d = a.damp[ql];
+            const f4 u = q[C];
+            const f4 dk = quad_damper(vzo, lzo, vxo, lxo, k);
+            const f4 ud = u * dk;
+            tb[b][row + 4][zq + 1] = u;
+            td[b][row + 4][zq + 1] = ud;
+            if (xh_on) {
+                if (xleft) td[b][xlrow][xhz + 1] = cu.xh * quad_damper(vzh, lzh, vxh, lxh, k);
+                else tb[b][xlrow][xhz + 1] = cu.xh;
+            }
+            if (zh_on) {
+                if (zbelow) td[b][zhrow + 4][zhcol] = cu.zh * quad_damper(vzb, lzb, vxb, lxb, k);
+                else tb[b][zhrow + 4][zhcol] = cu.zh;
+            }
+            __syncthreads();
+            const bool more = k + 1 < ke;
+            if (more) {
+                fetch(pf[b ^ 1], k + 1);
+                q[NEW] = bld4(vol_rsrc(cur, (long long)(k + 5) * NXZ, N), col);
+            }
+            if (any) {
+                const f4 zlo = td[b][row + 4][zq], zhi = tb[b][row + 4][zq + 2];
+                const f4 xm1 = td[b][row + 3][zq + 1], xp1 = tb[b][row + 5][zq + 1];
+                const f4 xm2 = td[b][row + 2][zq + 1], xp2 = tb[b][row + 6][zq + 1];
+                const f4 xm3 = td[b][row + 1][zq + 1], xp3 = tb[b][row + 7][zq + 1];
+                const f4 xm4 = td[b][row][zq + 1], xp4 = tb[b][row + 8][zq + 1];
+                // z-neighbours of cell j: lower index damped (own quad or the
+                // quad below), higher index undamped (own quad or above)
+                float zn[12], zp[12];
 #pragma unroll
-                for (int g = 0; g < G; g++) td[g][w][lane + 4] = ld(g, ql) * d;
-                if (xr < a.nxx) {
-                    const long long qr = pk + z + (long long)xr * a.nzz;
+                for (int e = 0; e < 4; e++) {
+                    zn[e] = zlo[e];
+                    zn[4 + e] = ud[e];
+                    zp[4 + e] = u[e];
+                    zp[8 + e] = zhi[e];
+                }
+                f4 res;
 #pragma unroll
-                    for (int g = 0; g < G; g++) tb[g][TX + 4 + w][lane + 4] = ld(g, qr);
+                for (int jj = 0; jj < 4; jj++) {
+                    const float uc = u[jj];
+                    float sx = -9.0f * (xm4[jj] + xp4[jj]);
+                    sx = sx + 128.0f * (xm3[jj] + xp3[jj]);
+                    sx = sx - 1008.0f * (xm2[jj] + xp2[jj]);
+                    sx = sx + 8064.0f * (xm1[jj] + xp1[jj]);
+                    sx = sx - 14350.0f * uc;
+                    float sy = -9.0f * (q[J % 10][jj] + q[(J + 8) % 10][jj]);
+                    sy = sy + 128.0f * (q[(J + 1) % 10][jj] + q[(J + 7) % 10][jj]);
+                    sy = sy - 1008.0f * (q[(J + 2) % 10][jj] + q[(J + 6) % 10][jj]);
+                    sy = sy + 8064.0f * (q[(J + 3) % 10][jj] + q[(J + 5) % 10][jj]);
+                    sy = sy - 14350.0f * uc;
+                    float sz = -9.0f * (zn[4 + jj - 4] + zp[4 + jj + 4]);
+                    sz = sz + 128.0f * (zn[4 + jj - 3] + zp[4 + jj + 3]);
+                    sz = sz - 1008.0f * (zn[4 + jj - 2] + zp[4 + jj + 2]);
+                    sz = sz + 8064.0f * (zn[4 + jj - 1] + zp[4 + jj + 1]);
+                    sz = sz - 14350.0f * uc;
+                    const float lap = (sx / a.den_x + sy / a.den_y) + sz / a.den_z;
+                    const float pas = cu.p[jj] * dk[jj];
+                    float f = cu.c[jj] * lap + 2.0f * uc;
+                    f = f - pas;
+                    res[jj] = f * dk[jj];
+                }
+                const auto Ro = vol_rsrc(out, (long long)k * NXZ, N);
+                if (all4) {
+                    bst4(Ro, col, res);
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++)
+                        if (comp[jj]) bst1(Ro, col + 4u * jj, res[jj]);
                 }
             }
-        }
-        // z halo: 4 cells below (damped) and 4 above (undamped) per tile row
-        if (threadIdx.x < 8 * TX) {
-            const int row = threadIdx.x >> 3, m = threadIdx.x & 7, x = x0 + row;
-            if (x < a.nxx) {
-                if (m < 4) {
-                    const long long q = pk + (z0 - 4 + m) + (long long)x * a.nzz;
-                    const float d = a.damp[q];
-#pragma unroll
-                    for (int g = 0; g < G; g++) td[g][row + 4][m] = ld(g, q) * d;
-                } else if (z0 + kWz + m - 4 < a.nzz) {
-                    const long long q = pk + (z0 + kWz + m - 4) + (long long)x * a.nzz;
-#pragma unroll
-                    for (int g = 0; g < G; g++) tb[g][row + 4][kWz + m] = ld(g, q);
-                }
-            }
-        }
-        __syncthreads();
-
-#pragma unroll
-        for (int r = 0; r < RX; r++) {
-            if (!comp[r]) continue;
-            const long long c = pk + col[r];
-            const float cv = a.cv[c];
-            const int ix = w * RX + r + 4, iz = lane + 4;
-#pragma unroll
-            for (int g = 0; g < G; g++) {
-                const float u = qc[g][r];
-                float sx = -9.0f * (td[g][ix - 4][iz] + tb[g][ix + 4][iz]);
-                sx = sx + 128.0f * (td[g][ix - 3][iz] + tb[g][ix + 3][iz]);
-                sx = sx - 1008.0f * (td[g][ix - 2][iz] + tb[g][ix + 2][iz]);
-                sx = sx + 8064.0f * (td[g][ix - 1][iz] + tb[g][ix + 1][iz]);
-                sx = sx - 14350.0f * u;
-                float sy = -9.0f * (qd[g][r][0] + qf[g][r][3]);
-                sy = sy + 128.0f * (qd[g][r][1] + qf[g][r][2]);
-                sy = sy - 1008.0f * (qd[g][r][2] + qf[g][r][1]);
-                sy = sy + 8064.0f * (qd[g][r][3] + qf[g][r][0]);
-                sy = sy - 14350.0f * u;
-                float sz = -9.0f * (td[g][ix][iz - 4] + tb[g][ix][iz + 4]);
-                sz = sz + 128.0f * (td[g][ix][iz - 3] + tb[g][ix][iz + 3]);
-                sz = sz - 1008.0f * (td[g][ix][iz - 2] + tb[g][ix][iz + 2]);
-                sz = sz + 8064.0f * (td[g][ix][iz - 1] + tb[g][ix][iz + 1]);
-                sz = sz - 14350.0f * u;
-                const float lap = (sx / a.den_x + sy / a.den_y) + sz / a.den_z;
-                float p = prev[g * a.N + c];
-                if (c == sid[g]) p = p + a.wprev;
-                const float pas = p * dk[r];
-                float f = cv * lap + 2.0f * u;
-                f = f - pas;
-                out[g * a.N + c] = f * dk[r];
-            }
-        }
-        // advance the y queue: the centre (damped) moves behind
-        const bool more = k + 1 < ke;
-#pragma unroll
-        for (int r = 0; r < RX; r++) {
-            const long long qn = pk + 5 * NXZ + col[r];
-#pragma unroll
-            for (int g = 0; g < G; g++) {
-                qd[g][r][0] = qd[g][r][1];
-                qd[g][r][1] = qd[g][r][2];
-                qd[g][r][2] = qd[g][r][3];
-                qd[g][r][3] = qc[g][r] * dk[r];
-                qc[g][r] = qf[g][r][0];
-                qf[g][r][0] = qf[g][r][1];
-                qf[g][r][1] = qf[g][r][2];
-                qf[g][r][2] = qf[g][r][3];
-                qf[g][r][3] = (more && in[r]) ? ld(g, qn) : 0.0f;
-            }
-        }
+            // the centre moves behind (damped); the loaded plane gets its wavelet
+            q[C] = ud;
+            if (more && k + 5 == ks) q[NEW] = inject(q[NEW], k + 5, col, a.wcur);
+        }, std::make_integer_sequence<int, 10>{});
     }
 }
 
@@ -449,10 +536,10 @@ namespace {
 
 struct WaveState {
     bool has_model = false;
-    int nx = 0, ny = 0, nz = 0, nb = 0, nxx = 0, nyy = 0, nzz = 0;
+    int nx = 0, ny = 0, nz = 0, nb = 0, nxx = 0, nyy = 0, nzz = 0, nzp = 0;
     float dx = 0, dy = 0, dz = 0, dt = 0, den[3] = {0, 0, 0};
     long long N = 0;
-    DevBuf cv, damp, fields, sid, rcell, rec, seis, pv, pi, scale, picks;
+    DevBuf cv, d1, fields, sid, rcell, rec, seis, pv, pi, scale, picks;
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -486,14 +573,15 @@ std::vector<long long> cells(const WaveState &w, const float *p, int n, const ch
                                        " lies outside the model");
         const int i = (int)(z / w.dz) + w.nb, j = (int)(x / w.dx) + w.nb,
                   k = (int)(y / w.dy) + w.nb;
-        c[q] = i + (long long)j * w.nzz + (long long)k * w.nxx * w.nzz;
+        c[q] = i + (long long)j * w.nzp + (long long)k * w.nxx * w.nzp;
     }
     return c;
 }
 
-template <int RX, int G> void launch_step(hipStream_t st, const StepArgs &a) {
-    dim3 grid(a.ntz * a.ntx, (a.nyy - 8 + a.ychunk - 1) / a.ychunk, 1);
-    hipLaunchKernelGGL((wave_step<RX, G>), grid, dim3(256), 0, st, a);
+template <int LZ> void launch_step(hipStream_t st, const StepArgs &a, int shots) {
+    const long long n = (long long)shots * a.ntz * a.ntx * a.nchunks;
+    ST_REQUIRE(n < (1LL << 31), ST_ERR_ARG, "acoustic model too large for one launch");
+    hipLaunchKernelGGL((wave_step<LZ>), dim3((unsigned)n), dim3(256), 0, st, a);
 }
 
 }  // namespace
@@ -501,7 +589,7 @@ template <int RX, int G> void launch_step(hipStream_t st, const StepArgs &a) {
 void wave_release(st_ctx *ctx) {
     if (!ctx->wave) return;
     auto *w = static_cast<WaveState *>(ctx->wave);
-    for (auto *b : {&w->cv, &w->damp, &w->fields, &w->sid, &w->rcell, &w->rec, &w->seis,
+    for (auto *b : {&w->cv, &w->d1, &w->fields, &w->sid, &w->rcell, &w->rec, &w->seis,
                     &w->pv, &w->pi, &w->scale, &w->picks})
         b->release();
     for (auto &e : w->ev)
@@ -520,29 +608,26 @@ void wave_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz, float
     w.nx = nx; w.ny = ny; w.nz = nz; w.nb = nb;
     w.nxx = nx + 2 * nb; w.nyy = ny + 2 * nb; w.nzz = nz + 2 * nb;
     w.dx = dx; w.dy = dy; w.dz = dz; w.dt = dt;
-    w.N = (long long)w.nxx * w.nyy * w.nzz;
+    w.nzp = (w.nzz + 31) & ~31;
+    w.N = (long long)w.nxx * w.nyy * w.nzp;
     w.den[0] = 5040.0f * (dx * dx);
     w.den[1] = 5040.0f * (dy * dy);
     w.den[2] = 5040.0f * (dz * dz);
     const long long M = (long long)nx * ny * nz;
+    // the 1D profile of dampers (acoustic.cpp:102-106), on the host libm;
+    // the kernels form the 2D and 3D entries from it
     std::vector<float> d1(nb), d2((size_t)nb * nb), d3((size_t)nb * nb * nb);
     acoustic_dampers(d1.data(), d2.data(), d3.data(), factor, nb);
-    DevBuf tmp, t1, t2, t3;
+    DevBuf tmp;
     upload(ctx, tmp, vel, M);
-    upload(ctx, t1, d1.data(), d1.size());
-    upload(ctx, t2, d2.data(), d2.size());
-    upload(ctx, t3, d3.data(), d3.size());
+    upload(ctx, w.d1, d1.data(), d1.size());
     w.cv.reserve(sizeof(float) * w.N);
-    w.damp.reserve(sizeof(float) * w.N);
     hipLaunchKernelGGL(wave_expand, dim3(grid_for(w.N)), dim3(256), 0, ctx->stream,
                        tmp.as<float>(), w.cv.as<float>(), nx, ny, nz, nb, w.nxx, w.nyy, w.nzz,
-                       dt * dt);
-    hipLaunchKernelGGL(wave_damper_volume, dim3(grid_for(w.N)), dim3(256), 0, ctx->stream,
-                       t1.as<float>(), t2.as<float>(), t3.as<float>(), w.damp.as<float>(), nb,
-                       w.nxx, w.nyy, w.nzz);
+                       w.nzp, dt * dt);
     ST_HIP(hipGetLastError());
     ST_HIP(hipStreamSynchronize(ctx->stream));
-    tmp.release(); t1.release(); t2.release(); t3.release();
+    tmp.release();
     w.has_model = true;
 }
 
@@ -583,11 +668,38 @@ void wave_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz,
     if (picks_out) ST_REQUIRE(nt <= kMaxPickSamples, ST_ERR_ARG, "picker: nt above 8192");
     const std::vector<long long> sc = cells(w, src_xyz, nshot, "source");
     const std::vector<long long> rc = cells(w, rcv_xyz, nrec, "receiver");
-    int G = group > 0 ? group : (nshot >= 2 ? 2 : 1);
-    ST_REQUIRE(G == 1 || G == 2 || G == 4, ST_ERR_ARG, "shots per launch must be 1, 2 or 4");
-    G = std::min(G, nshot == 1 ? 1 : G);
-    const int ngroups = (nshot + G - 1) / G;
+    ST_REQUIRE(group >= 0, ST_ERR_ARG, "shots per launch must be >= 0");
     const long long N = w.N;
+    StepArgs a{};
+    a.cv = w.cv.as<float>();
+    a.d1 = w.d1.as<float>();
+    a.nb = w.nb;
+    a.N = N;
+    a.nzz = w.nzz; a.nzp = w.nzp; a.nzr = (w.nzz + 3) & ~3; a.nxx = w.nxx; a.nyy = w.nyy;
+    // z tiles of 64 cells (16 lanes of four cells per x row), or of 32 when
+    // that leaves markedly fewer idle lanes; tiles start at z = 0 (line
+    // aligned) and cover the interior 4 <= z < nzz - 4
+    const int nzi = w.nzz - 8, nxi = w.nxx - 8, nyi = w.nyy - 8;
+    auto eff = [&](int tz) { return (double)nzi / (double)(((w.nzz - 4 + tz - 1) / tz) * tz); };
+    const int LZ = eff(32) > eff(64) + 0.05 ? 8 : 16;
+    const int TZ = 4 * LZ, TXT = kWaves * (64 / LZ);
+    a.ntz = (w.nzz - 4 + TZ - 1) / TZ;
+    a.ntx = (nxi + TXT - 1) / TXT;
+    const long long tiles = (long long)a.ntz * a.ntx;
+    // shots per launch: enough workgroups (about 2048, 8 per CU) with chunks
+    // of at least 32 planes; three wavefields per shot, at most 24 GiB
+    int G = group;
+    if (G == 0) {
+        const long long wg1 = tiles * std::max(1, nyi / 32);
+        G = (int)std::max<long long>(1, (2048 + wg1 - 1) / wg1);
+    }
+    G = std::min(G, nshot);
+    G = (int)std::max<long long>(1, std::min<long long>(G, (24LL << 30) / (12 * N)));
+    const long long chunks = std::max<long long>(1, (2048 + G * tiles - 1) / (G * tiles));
+    a.ychunk = (int)std::min<long long>(nyi, std::max<long long>(32, (nyi + chunks - 1) / chunks));
+    a.nchunks = (nyi + a.ychunk - 1) / a.ychunk;
+    a.den_x = w.den[0]; a.den_y = w.den[1]; a.den_z = w.den[2];
+    const int ngroups = (nshot + G - 1) / G;
     w.fields.reserve(sizeof(float) * 3 * G * N);
     w.rcell.reserve(sizeof(long long) * nrec);
     w.sid.reserve(sizeof(long long) * G);
@@ -598,27 +710,11 @@ void wave_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz,
                           hipMemcpyHostToDevice, ctx->stream));
     float *F[3] = {w.fields.as<float>(), w.fields.as<float>() + G * N,
                    w.fields.as<float>() + 2 * G * N};
-    StepArgs a{};
-    a.cv = w.cv.as<float>();
-    a.damp = w.damp.as<float>();
     a.sid = w.sid.as<long long>();
-    a.N = N;
-    a.nzz = w.nzz; a.nxx = w.nxx; a.nyy = w.nyy;
-    const int RX = G == 4 ? 2 : 4;
-    a.ntz = (w.nzz - 8 + kWz - 1) / kWz;
-    a.ntx = (w.nxx - 8 + kWaves * RX - 1) / (kWaves * RX);
-    // about 2048 workgroups (8 per CU): split y into chunks when the tiles of
-    // a plane are fewer; a chunk re-reads 8 planes of the queue
-    const long long tiles = (long long)a.ntz * a.ntx;
-    const long long chunks = std::max<long long>(1, (2048 + tiles - 1) / tiles);
-    a.ychunk = (int)std::max<long long>(16, (w.nyy - 8 + chunks - 1) / chunks);
-    a.ychunk = std::min(a.ychunk, w.nyy - 8);
-    a.den_x = w.den[0]; a.den_y = w.den[1]; a.den_z = w.den[2];
     ST_HIP(hipEventRecord(w.ev[0], ctx->stream));
     for (int gi = 0; gi < ngroups; gi++) {
-        std::vector<long long> sid(G, -1);
-        for (int g = 0; g < G && gi * G + g < nshot; g++) sid[g] = sc[gi * G + g];
-        ST_HIP(hipMemcpyAsync(w.sid.ptr, sid.data(), sizeof(long long) * G,
+        const int gs = std::min(G, nshot - gi * G);
+        ST_HIP(hipMemcpyAsync(w.sid.ptr, sc.data() + (size_t)gi * G, sizeof(long long) * gs,
                               hipMemcpyHostToDevice, ctx->stream));
         ST_HIP(hipMemsetAsync(w.fields.ptr, 0, sizeof(float) * 3 * G * N, ctx->stream));
         int ip = 0, ic = 1, io = 2;   // U_pas source, U_pre, U_fut
@@ -628,18 +724,16 @@ void wave_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz,
             a.out = F[io];
             a.wcur = wavelet[t];
             a.wprev = t > 0 ? wavelet[t - 1] : 0.0f;
-            if (G == 1) launch_step<4, 1>(ctx->stream, a);
-            else if (G == 2) launch_step<4, 2>(ctx->stream, a);
-            else launch_step<2, 4>(ctx->stream, a);
-            hipLaunchKernelGGL(wave_record, dim3((nrec + 255) / 256, G), dim3(256), 0, ctx->stream,
+            if (LZ == 16) launch_step<16>(ctx->stream, a, gs);
+            else launch_step<8>(ctx->stream, a, gs);
+            hipLaunchKernelGGL(wave_record, dim3((nrec + 255) / 256, gs), dim3(256), 0, ctx->stream,
                                F[io], N, w.rcell.as<long long>(), nrec, G, w.rec.as<float>(), t);
             const int tmp = ip;
             ip = ic; ic = io; io = tmp;
         }
         ST_HIP(hipGetLastError());
-        hipLaunchKernelGGL(wave_transpose, dim3((nt + 31) / 32, (nrec + 31) / 32, G), dim3(256), 0,
+        hipLaunchKernelGGL(wave_transpose, dim3((nt + 31) / 32, (nrec + 31) / 32, gs), dim3(256), 0,
                            ctx->stream, w.rec.as<float>(), w.seis.as<float>(), nt, G, nrec);
-        const int gs = std::min(G, nshot - gi * G);
         if (picks_out)
             pick_device(ctx, w, w.seis.as<float>(), gs, nrec, nt, w.dt, pick_params[0],
                         pick_params[1], pick_params[2], w.picks.as<float>());
@@ -652,8 +746,9 @@ void wave_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz,
                                   sizeof(float) * (size_t)gs * nrec, hipMemcpyDeviceToHost,
                                   ctx->stream));
         if (ufut_out && gi == ngroups - 1)   // U_fut of the last shot (wave_main.cpp:160)
-            ST_HIP(hipMemcpyAsync(ufut_out, F[ic] + (size_t)(gs - 1) * N, sizeof(float) * N,
-                                  hipMemcpyDeviceToHost, ctx->stream));
+            ST_HIP(hipMemcpy2DAsync(ufut_out, sizeof(float) * w.nzz, F[ic] + (size_t)(gs - 1) * N,
+                                    sizeof(float) * w.nzp, sizeof(float) * w.nzz,
+                                    (size_t)w.nxx * w.nyy, hipMemcpyDeviceToHost, ctx->stream));
     }
     ST_HIP(hipEventRecord(w.ev[1], ctx->stream));
     ST_HIP(hipStreamSynchronize(ctx->stream));

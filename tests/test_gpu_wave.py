@@ -35,6 +35,7 @@ def _sources(c, n, seed=0):
     (dict(seed=4, nb=5, spacing=(10.0, 12.5, 7.5)), 2),
     (dict(seed=5, nb=4, nt=80), 4),
     (dict(seed=6, nx=70, ny=9, nz=90, nt=90), 2),          # several z and x tiles
+    (dict(seed=7, nx=40, ny=11, nz=112, nt=70), 0),        # 64-cell z tiles
 ])
 def test_seismograms_bit_exact(ctx, case, group):
     c = orc.acoustic_case(**case)
@@ -123,4 +124,4 @@ def test_bad_arguments(ctx):
     with pytest.raises(StError):
         ctx.acoustic_shots(c["wavelet"], np.array([[-5.0, 10.0, 10.0]]), c["rcv"])
     with pytest.raises(StError):
-        ctx.acoustic_shots(c["wavelet"], c["src"][None], c["rcv"], group=3)
+        ctx.acoustic_shots(c["wavelet"], c["src"][None], c["rcv"], group=-1)

@@ -92,3 +92,42 @@ def test_powf_square_folds_to_the_product():
     picks = np.zeros(1, np.float32)
     orc.oracle().or_auto_pick(P(seis), 1, nt, float(dt), 0.01, 1e-15, 0.0, P(picks))
     assert picks[0] == np.float32(5000 - 10 + 1) * dt
+
+
+def _damper_formula(d1, lz, lx, ly):
+    """The damper of a cell from the 1D profile alone, in the reference's
+    accumulation order (acoustic.cpp:108-136): one layered axis d1[l]; two
+    (d1[a] + d1[b]) - 1; three ((d1z + d1x) + (d1x + d1y)) + (d1z + d1y) - 5."""
+    f = np.float32
+    ins = [l >= 0 for l in (lz, lx, ly)]
+    v = [d1[l] if l >= 0 else f(0) for l in (lz, lx, ly)]
+    n = sum(ins)
+    if n == 0:
+        return f(1)
+    if n == 1:
+        return v[ins.index(True)]
+    if n == 2:
+        a, b = (v[i] for i in range(3) if ins[i])
+        return f(f(a + b) - f(1))
+    dz, dx, dy = v
+    return f(f(f(f(dz + dx) + f(dx + dy)) + f(dz + dy)) - f(5))
+
+
+@pytest.mark.parametrize("nb,factor", [(8, 0.028), (13, 0.02), (50, 0.0045)])
+def test_damper_from_profile_equals_tables(nb, factor):
+    """The device computes a cell's damper from d1 alone; every entry of
+    the 1D, 2D and 3D tables must come out the same, bit for bit."""
+    d1, d2, d3 = (np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3))
+    orc.oracle().or_acoustic_dampers(P(d1), P(d2), P(d3), factor, nb)
+    for l in range(nb):
+        assert _damper_formula(d1, l, -1, -1) == d1[l]
+    for a in range(nb):
+        for b in range(nb):
+            assert _damper_formula(d1, -1, a, b) == d2[a + b * nb]
+            assert _damper_formula(d1, a, -1, b) == d2[a + b * nb]
+            assert _damper_formula(d1, a, b, -1) == d2[a + b * nb]
+    rng = np.random.default_rng(nb)
+    idx = rng.integers(0, nb, (4000, 3)) if nb > 16 else np.array(
+        [(i, j, k) for i in range(nb) for j in range(nb) for k in range(nb)])
+    for i, j, k in idx:
+        assert _damper_formula(d1, i, j, k) == d3[i + j * nb + k * nb * nb], (i, j, k)
