@@ -225,6 +225,83 @@ def tomography_leg(with_cpu):
 
 
 
+def synthesis_leg(with_cpu, traffic_file, nt=3001):
+    """Data synthesis at the reference's wave_main.cpp / auto_picking.cpp
+    sizes: 701x501x101 at 10 m with 50 absorbing cells per side (801x601x201
+    padded), nt 3001 at 1 ms, 14,000 receivers on a 140x100 grid; two shots
+    modelled and auto-picked on the device (synthetic layered model, 40 Hz
+    Ricker).  Beside it the reference's own serial loop (acoustic.cpp built
+    from its source) on the same padded volume for two steps, one core, and
+    the picker's C restatement over one shot's file."""
+    from seismic_tomography_3d_amd import Context, synth
+    from seismic_tomography_3d_amd.api import ricker
+    nx, ny, nz, h, nb, dt = 701, 501, 101, 10.0, 50, 1e-3
+    vel = synth.layered_model(nx, ny, nz, h, n_layers=5, n_anom=3, seed=2)
+    w = ricker(nt, dt, 40.0, 0.1)
+    gx, gy = np.meshgrid(np.linspace(500.0, 6500.0, 140), np.linspace(500.0, 4500.0, 100))
+    rcv = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 10.0)], 1).astype(np.float32)
+    src = np.array([[3500.0, 2500.0, 500.0], [1500.0, 1200.0, 500.0]], np.float32)
+    ctx = Context(0)
+    ctx.acoustic_set_model(vel, h, nb=nb, factor=0.0045, dt=dt)
+    cells = (nx + 2 * nb - 8) * (ny + 2 * nb - 8) * (nz + 2 * nb - 8)
+    # the step alone (plus its receiver recording), one shot, 200 steps
+    ns = min(200, nt)
+    ctx.acoustic_shots(w[:min(20, nt)], src[:1], rcv, group=1, return_seis=False)
+    _, _, ms_s = ctx.acoustic_shots(w[:ns], src[:1], rcv, group=1, return_seis=False)
+    step_ms = ms_s / ns
+    t0 = time.perf_counter()
+    picks, dev_ms = ctx.acoustic_picks(w, src, rcv)
+    wall = time.perf_counter() - t0
+    seis, _, _ = ctx.acoustic_shots(w[:min(600, nt)], src[:1], rcv[:2000], return_seis=True)
+    t1 = time.perf_counter()
+    ctx.auto_pick(seis[0], dt)
+    pick_ms = 1e3 * (time.perf_counter() - t1)
+    ctx.close()
+    achieved = 16.0 * cells / (step_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    out = {"workload": "acoustic modelling + auto-picking, 701x501x101 at 10 m, nb 50 "
+                       f"(801x601x201 padded), nt {nt} at 1 ms, 14,000 receivers, 2 shots",
+           "value": len(src) / (dev_ms * 1e-3), "unit": "shots/s (modelled and picked)",
+           "ms_per_shot": dev_ms / len(src), "wall_s": wall, "step_ms": step_ms,
+           "cells_per_step": cells, "cell_updates_per_s": cells / (step_ms * 1e-3),
+           "picked_fraction": float(np.count_nonzero(picks)) / picks.size,
+           "picker_ms_2000x600_from_host": pick_ms,
+           "roofline": {"kernel": "wave_step", "bound": "hbm", "achieved": achieved,
+                        "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                        "traffic": traffic,
+                        "bytes_per_cell": "16 (U_pre, U_pas, V^2 dt^2 read; U_fut written)"}}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        from oracle_lib import P
+        v = np.ascontiguousarray(vel, np.float32)
+        rx, ry, rz = (np.ascontiguousarray(rcv[:2, i]) for i in range(3))
+        kind = "reference" if orc.ref_available() else "port"
+        fn = orc.ref().ref_acoustic_shot if kind == "reference" else orc.oracle().or_acoustic_shot
+        ts = []
+        for n in (1, 2):
+            sg = np.zeros(2 * n, np.float32)
+            t0 = time.perf_counter()
+            fn(P(v), nx, ny, nz, h, h, h, nb, 0.0045, P(w), n, dt, *(float(c) for c in src[0]),
+               P(rx), P(ry), P(rz), 2, P(sg), None)
+            ts.append(time.perf_counter() - t0)
+        cpu_step = max(ts[1] - ts[0], 1e-9)
+        out["cpu_baseline"] = {"value": cells / cpu_step, "unit": "cell-updates/s", "cores": 1,
+                               "kind": kind, "step_s": cpu_step,
+                               "sample": "2 time steps of one shot on the full 801x601x201 "
+                                         "padded volume (step time = t(2 steps) - t(1 step))"}
+        out["speedup_vs_cpu_reference"] = out["cell_updates_per_s"] / out["cpu_baseline"]["value"]
+        s2 = np.ascontiguousarray(seis[0])
+        pk = np.zeros(s2.shape[0], np.float32)
+        t0 = time.perf_counter()
+        orc.oracle().or_auto_pick(P(s2), s2.shape[0], s2.shape[1], dt, 0.01, 1e-15, 0.045, P(pk))
+        out["picker_cpu_port_ms_2000x600"] = 1e3 * (time.perf_counter() - t0)
+    return out
+
+
 def calibrate_copy(mib):
     """A device-to-device copy of known size (torch's copy kernel), three
     times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch
@@ -242,6 +319,10 @@ def calibrate_copy(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--synthesis-only", action="store_true",
+                    help="run only the data-synthesis leg (acoustic modelling + picking)")
+    ap.add_argument("--synthesis-nt", type=int, default=3001,
+                    help="time steps of the data-synthesis leg (3001 as the reference)")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=256)
@@ -273,6 +354,12 @@ def main():
 
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.synthesis_only:
+        leg = synthesis_leg(not a.no_cpu,
+                            os.path.join(ROOT, "profiles", "wave_step_801x601x201_traffic.json"),
+                            a.synthesis_nt)
+        print(json.dumps({"data_synthesis": leg}), flush=True)
         return
 
     from seismic_tomography_3d_amd import Context, synth, tune
@@ -416,6 +503,9 @@ def main():
     if not a.no_extra:
         out["tomography_iteration"] = tomography_leg(not a.no_cpu)
         log(f"tomography_iteration: {out['tomography_iteration']}")
+        out["data_synthesis"] = synthesis_leg(
+            not a.no_cpu, os.path.join(ROOT, "profiles", "wave_step_801x601x201_traffic.json"))
+        log(f"data_synthesis: {out['data_synthesis']}")
     print(json.dumps(out), flush=True)
 
 

@@ -225,6 +225,10 @@ int st_nn_gemm(st_ctx *ctx, const float *A, const float *B, float *C, int M,
  * Replaces dampers (acoustic.cpp:100-137). */
 int st_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
 
+/* Ricker wavelet on the host: out[t] for t*dt - delay, peak frequency
+ * parameter fmax.  Replaces rickerGeneration (acoustic.cpp:81-98). */
+int st_ricker(float delay, float fmax, float dt, int nt, float *out);
+
 /* Velocity model vel[ny][nx][nz] (z fastest, m/s), spacings, nb absorbing
  * cells per side (>= 4), damping factor and time step.  Replaces
  * wave_main.cpp:10-48 (expand, dampers). */

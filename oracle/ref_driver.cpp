@@ -238,6 +238,13 @@ int ref_acoustic_shot(const float *vel, int nx, int ny, int nz, float dx, float 
     return 0;
 }
 
+// The reference's Ricker wavelet (acoustic.cpp:81-98).
+void ref_ricker(float delay, float fmax, float dt, int nt, float *out) {
+    float *r = rickerGeneration(delay, fmax, dt, nt);
+    std::copy(r, r + nt, out);
+    delete[] r;
+}
+
 // The reference's damping profiles (acoustic.cpp:100-137).
 void ref_dampers(float *d1, float *d2, float *d3, float factor, int nb) {
     std::fill(d1, d1 + nb, 0.f);

@@ -101,6 +101,8 @@ SIGNATURES = [
                                   ctypes.c_int, ctypes.c_int]),
     ("st_acoustic_dampers", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_float,
                                            ctypes.c_int]),
+    ("st_ricker", ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                 c_float_p]),
     ("st_acoustic_set_model", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                              ctypes.c_float, ctypes.c_float, ctypes.c_int,

@@ -149,15 +149,15 @@ class Context:
                                                    int(nb), factor, dt))
         self._acoustic = (nx, ny, nz, int(nb))
 
-    def acoustic_shots(self, wavelet, src, rcv, group=0, return_ufut=False):
+    def acoustic_shots(self, wavelet, src, rcv, group=0, return_ufut=False, return_seis=True):
         """Seismograms [nshot, nrec, nt] of the acoustic modelling (and the
-        last shot's final padded U_fut [nyy, nxx, nzz]); returns (seis, ufut
-        or None, device ms)."""
+        last shot's final padded U_fut [nyy, nxx, nzz]); returns (seis or
+        None, ufut or None, device ms)."""
         w = _f32(wavelet, (-1,))
         src = _f32(src, (-1, 3))
         rcv = _f32(rcv, (-1, 3))
         ns, nr, nt = src.shape[0], rcv.shape[0], w.shape[0]
-        seis = np.empty((ns, nr, nt), np.float32)
+        seis = np.empty((ns, nr, nt), np.float32) if return_seis else None
         uf = None
         if return_ufut:
             nx, ny, nz, nb = self._acoustic
@@ -351,3 +351,12 @@ def acoustic_dampers(factor, nb):
     if rc != 0:
         raise ValueError(f"st_acoustic_dampers: status {rc}")
     return d1, d2, d3
+
+
+def ricker(nt, dt, fmax, delay):
+    """The reference's Ricker wavelet (made on the host)."""
+    out = np.zeros(int(nt), np.float32)
+    rc = _lib.load().st_ricker(delay, fmax, dt, int(nt), _ptr(out))
+    if rc != 0:
+        raise ValueError(f"st_ricker: status {rc}")
+    return out

@@ -32,6 +32,7 @@ void tomo_release(st_ctx *ctx);
 void surrogate_release(st_ctx *ctx);
 void wave_release(st_ctx *ctx);
 void acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
+void acoustic_ricker(float delay, float fmax, float dt, int nt, float *out);
 void wave_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz, float dx, float dy,
                     float dz, int nb, float factor, float dt);
 void wave_shots(st_ctx *ctx, const float *wavelet, int nt, const float *src_xyz, int nshot,
@@ -694,6 +695,12 @@ int st_model_update(st_ctx *ctx, const float *xdm, int smooth, int samples, floa
 int st_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb) {
     if (!d1 || !d2 || !d3 || nb <= 0) return ST_ERR_ARG;
     acoustic_dampers(d1, d2, d3, factor, nb);
+    return ST_OK;
+}
+
+int st_ricker(float delay, float fmax, float dt, int nt, float *out) {
+    if (!out || nt <= 0) return ST_ERR_ARG;
+    acoustic_ricker(delay, fmax, dt, nt, out);
     return ST_OK;
 }
 

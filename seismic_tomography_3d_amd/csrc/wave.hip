@@ -532,6 +532,20 @@ void acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb) {
     for (size_t q = 0; q < (size_t)nb * nb * nb; q++) d3[q] -= 5.0f;
 }
 
+// rickerGeneration (acoustic.cpp:81-98), on the host, in the same mixed
+// precision: the last factor of the exponent is a double (pow of pi)
+void acoustic_ricker(float delay, float fmax, float dt, int nt, float *out) {
+    const float pi = 4.0f * atanf(1.0f);
+    const float fc = fmax / (3.0f * sqrtf(pi));
+    for (int t = 0; t < nt; t++) {
+        const float u = (float)t * dt - delay;
+        const float aux1 = 1.0f - 2.0f * pi * (u * u) * (fc * fc) * (pi * pi);
+        const double e = (double)(-pi * (u * u) * (fc * fc)) * ((double)pi * (double)pi);
+        const float aux2 = expf((float)e);
+        out[t] = aux1 * aux2;
+    }
+}
+
 namespace {
 
 struct WaveState {

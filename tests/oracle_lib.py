@@ -103,6 +103,8 @@ def ref():
                                         fp, fp, fp, I, fp, fp]
         r.ref_acoustic_shot.restype = I
         r.ref_dampers.argtypes = [fp, fp, fp, F, I]
+        r.ref_ricker.argtypes = [F, F, F, I, fp]
+        r.ref_ricker.restype = None
         r.ref_dampers.restype = None
         _R = r
     return _R

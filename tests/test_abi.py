@@ -57,3 +57,23 @@ def test_acoustic_dampers_host_entry_matches_oracle(nb, factor):
         r = [np.zeros(n, np.float32) for n in (nb, nb * nb, nb ** 3)]
         orc.ref().ref_dampers(*(P(x) for x in r), factor, nb)
         assert np.array_equal(d3.ravel(), r[2])
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libst_hip.so not built")
+@pytest.mark.parametrize("nt,dt,fmax,delay", [(3001, 1e-3, 40.0, 0.1), (500, 5e-4, 25.0, 0.05)])
+def test_ricker_host_entry(nt, dt, fmax, delay):
+    """st_ricker (host only) equals the reference's rickerGeneration bit for
+    bit where the reference build is present; in any case it peaks at 1 at
+    t = delay and is symmetric about it."""
+    import numpy as np
+    import oracle_lib as orc
+    from oracle_lib import P
+    from seismic_tomography_3d_amd.api import ricker
+    w = ricker(nt, dt, fmax, delay)
+    k = int(round(delay / dt))
+    assert abs(w[k] - 1.0) < 1e-6 and w.max() == w[k]
+    assert np.allclose(w[k - 20:k], w[k + 20:k:-1], atol=1e-5)
+    if orc.ref_available():
+        r = np.zeros(nt, np.float32)
+        orc.ref().ref_ricker(delay, fmax, dt, nt, P(r))
+        assert np.array_equal(w, r)
