@@ -1310,3 +1310,210 @@ void or_auto_pick(const float *seis, int ntraces, int nt, float dt, float window
     }
     free(x); free(S);
 }
+
+
+/* ====================================================================== */
+/* eikonal: Podvin & Lecomte (classic.cpp)                                 */
+/* ====================================================================== */
+
+/* Offsets (dz, dx, dy) of the operators of classic.cpp:301-1773, in the
+ * reference's order.  S cells are the voxels whose lower corner is the
+ * given node. */
+typedef struct { signed char t[3], s[4][3]; } PodHead;          /* :301-341 */
+typedef struct { signed char t[3], s[2][3]; } PodDiff;          /* :343-395 */
+typedef struct { signed char t[3], s[3]; } PodCorner;           /* :397-429 */
+typedef struct { signed char s[2][3], tt[2][2][3]; } PodQuad;   /* :431-765: Sref, (T1, T2) x 2 */
+typedef struct { signed char s[3], f[3][4][3]; } PodOct;        /* :767-1773: Sref, faces (M,N,P,Q) */
+
+static const PodHead POD_HEAD[6] = {
+    {{0,-1,0}, {{0,-1,0}, {-1,-1,0}, {0,-1,-1}, {-1,-1,-1}}},
+    {{0,1,0}, {{0,0,0}, {-1,0,0}, {0,0,-1}, {-1,0,-1}}},
+    {{0,0,-1}, {{0,0,-1}, {0,-1,-1}, {-1,0,-1}, {-1,-1,-1}}},
+    {{0,0,1}, {{0,0,0}, {-1,0,0}, {0,-1,0}, {-1,-1,0}}},
+    {{-1,0,0}, {{-1,0,0}, {-1,-1,0}, {-1,0,-1}, {-1,-1,-1}}},
+    {{1,0,0}, {{0,0,0}, {0,-1,0}, {0,0,-1}, {0,-1,-1}}},
+};
+static const PodDiff POD_DIFF[12] = {
+    {{-1,-1,0}, {{-1,-1,0}, {-1,-1,-1}}}, {{-1,1,0}, {{-1,0,0}, {-1,0,-1}}},
+    {{1,-1,0}, {{0,-1,0}, {0,-1,-1}}},    {{1,1,0}, {{0,0,0}, {0,0,-1}}},
+    {{-1,0,-1}, {{-1,0,-1}, {-1,-1,-1}}}, {{-1,0,1}, {{-1,0,0}, {-1,-1,0}}},
+    {{1,0,-1}, {{0,0,-1}, {0,-1,-1}}},    {{1,0,1}, {{0,0,0}, {0,-1,0}}},
+    {{0,-1,-1}, {{0,-1,-1}, {-1,-1,-1}}}, {{0,-1,1}, {{0,-1,0}, {-1,-1,0}}},
+    {{0,1,-1}, {{0,0,-1}, {-1,0,-1}}},    {{0,1,1}, {{0,0,0}, {-1,0,0}}},
+};
+static const PodCorner POD_CORNER[8] = {
+    {{-1,-1,-1}, {-1,-1,-1}}, {{-1,-1,1}, {-1,-1,0}}, {{1,-1,-1}, {0,-1,-1}},
+    {{1,-1,1}, {0,-1,0}},     {{-1,1,-1}, {-1,0,-1}}, {{-1,1,1}, {-1,0,0}},
+    {{1,1,-1}, {0,0,-1}},     {{1,1,1}, {0,0,0}},
+};
+static const PodQuad POD_QUAD[12] = {
+    {{{-1,-1,0}, {-1,-1,-1}}, {{{0,-1,0}, {-1,-1,0}}, {{-1,0,0}, {-1,-1,0}}}},
+    {{{0,-1,0}, {0,-1,-1}}, {{{0,-1,0}, {1,-1,0}}, {{1,0,0}, {1,-1,0}}}},
+    {{{0,0,0}, {0,0,-1}}, {{{1,0,0}, {1,1,0}}, {{0,1,0}, {1,1,0}}}},
+    {{{-1,0,0}, {-1,0,-1}}, {{{0,1,0}, {-1,1,0}}, {{-1,0,0}, {-1,1,0}}}},
+    {{{-1,0,-1}, {-1,-1,-1}}, {{{0,0,-1}, {-1,0,-1}}, {{-1,0,0}, {-1,0,-1}}}},
+    {{{0,0,-1}, {0,-1,-1}}, {{{0,0,-1}, {1,0,-1}}, {{1,0,0}, {1,0,-1}}}},
+    {{{0,0,0}, {0,-1,0}}, {{{1,0,0}, {1,0,1}}, {{0,0,1}, {1,0,1}}}},
+    {{{-1,0,0}, {-1,-1,0}}, {{{0,0,1}, {-1,0,1}}, {{-1,0,0}, {-1,0,1}}}},
+    {{{0,-1,-1}, {-1,-1,-1}}, {{{0,-1,0}, {0,-1,-1}}, {{0,0,-1}, {0,-1,-1}}}},
+    {{{0,-1,0}, {-1,-1,0}}, {{{0,-1,0}, {0,-1,1}}, {{0,0,1}, {0,-1,1}}}},
+    {{{0,0,0}, {-1,0,0}}, {{{0,0,1}, {0,1,1}}, {{0,1,0}, {0,1,1}}}},
+    {{{0,0,-1}, {-1,0,-1}}, {{{0,1,0}, {0,1,-1}}, {{0,0,-1}, {0,1,-1}}}},
+};
+static const PodOct POD_OCT[8] = {
+    {{-1,-1,-1}, {{{-1,-1,-1}, {-1,-1,0}, {-1,0,-1}, {-1,0,0}}, {{-1,-1,-1}, {-1,-1,0}, {0,-1,-1}, {0,-1,0}}, {{-1,-1,-1}, {0,-1,-1}, {-1,0,-1}, {0,0,-1}}}},
+    {{-1,0,-1}, {{{-1,1,-1}, {-1,0,-1}, {-1,1,0}, {-1,0,0}}, {{-1,1,-1}, {-1,1,0}, {0,1,-1}, {0,1,0}}, {{-1,1,-1}, {-1,0,-1}, {0,1,-1}, {0,0,-1}}}},
+    {{-1,0,0}, {{{-1,1,1}, {-1,1,0}, {-1,0,1}, {-1,0,0}}, {{-1,1,1}, {0,1,1}, {-1,1,0}, {0,1,0}}, {{-1,1,1}, {-1,0,1}, {0,1,1}, {0,0,1}}}},
+    {{-1,-1,0}, {{{-1,-1,1}, {-1,0,1}, {-1,-1,0}, {-1,0,0}}, {{-1,-1,1}, {-1,-1,0}, {0,-1,1}, {0,-1,0}}, {{-1,-1,1}, {0,-1,1}, {-1,0,1}, {0,0,1}}}},
+    {{0,-1,-1}, {{{1,-1,-1}, {1,0,-1}, {1,-1,0}, {1,0,0}}, {{1,-1,-1}, {1,-1,0}, {0,-1,-1}, {0,-1,0}}, {{1,-1,-1}, {0,-1,-1}, {1,0,-1}, {0,0,-1}}}},
+    {{0,0,-1}, {{{1,1,-1}, {1,1,0}, {1,0,-1}, {1,0,0}}, {{1,1,-1}, {0,1,-1}, {1,1,0}, {0,1,0}}, {{1,1,-1}, {1,0,-1}, {0,1,-1}, {0,0,-1}}}},
+    {{0,-1,0}, {{{1,-1,1}, {1,-1,0}, {1,0,1}, {1,0,0}}, {{1,-1,1}, {0,-1,1}, {1,-1,0}, {0,-1,0}}, {{1,-1,1}, {1,0,1}, {0,-1,1}, {0,0,1}}}},
+    {{0,0,0}, {{{1,1,1}, {1,0,1}, {1,1,0}, {1,0,0}}, {{1,1,1}, {1,1,0}, {0,1,1}, {0,1,0}}, {{1,1,1}, {0,1,1}, {1,0,1}, {0,0,1}}}},
+};
+
+static inline float pod_min(float a, float b) { return b < a ? b : a; }   /* std::min */
+
+/* the lowest of T and every operator's arrival at cell c (classic.cpp:297-1773) */
+static float pod_update(const float *T, const float *S, long c, long NZ, long NXZ, float h,
+                        float sqrt2, float sqrt3)
+{
+#define OFF(o) ((long)(o)[0] + (long)(o)[1] * NZ + (long)(o)[2] * NXZ)
+    float lowest = T[c], Tijk;
+    for (int q = 0; q < 6; q++) {          /* 1D head waves */
+        const PodHead *o = &POD_HEAD[q];
+        const float m = pod_min(S[c + OFF(o->s[0])], pod_min(S[c + OFF(o->s[1])],
+                        pod_min(S[c + OFF(o->s[2])], S[c + OFF(o->s[3])])));
+        Tijk = T[c + OFF(o->t)] + h * m;
+        if (Tijk < lowest) lowest = Tijk;
+    }
+    for (int q = 0; q < 12; q++) {         /* edge diffractions */
+        const PodDiff *o = &POD_DIFF[q];
+        Tijk = T[c + OFF(o->t)] + h * sqrt2 * pod_min(S[c + OFF(o->s[0])], S[c + OFF(o->s[1])]);
+        if (Tijk < lowest) lowest = Tijk;
+    }
+    for (int q = 0; q < 8; q++) {          /* corner diffractions */
+        const PodCorner *o = &POD_CORNER[q];
+        Tijk = T[c + OFF(o->t)] + h * sqrt3 * S[c + OFF(o->s)];
+        if (Tijk < lowest) lowest = Tijk;
+    }
+    for (int q = 0; q < 12; q++) {         /* 2D operators */
+        const PodQuad *o = &POD_QUAD[q];
+        const float Sref = pod_min(S[c + OFF(o->s[0])], S[c + OFF(o->s[1])]);
+        for (int e = 0; e < 2; e++) {
+            const float T1 = T[c + OFF(o->tt[e][0])], T2 = T[c + OFF(o->tt[e][1])];
+            if ((T1 - T2) > 0.0f && (T1 - T2) < h * Sref / sqrt2) {
+                Tijk = T1 + sqrtf(h * h * Sref * Sref - (T1 - T2) * (T1 - T2));
+                if (Tijk < lowest) lowest = Tijk;
+            }
+        }
+    }
+    for (int q = 0; q < 8; q++) {          /* 3D operators */
+        const PodOct *o = &POD_OCT[q];
+        const float Sref = S[c + OFF(o->s)];
+        const float hs2 = h * h * Sref * Sref;
+        for (int f = 0; f < 3; f++) {
+            const float M = T[c + OFF(o->f[f][0])], N = T[c + OFF(o->f[f][1])],
+                        P = T[c + OFF(o->f[f][2])], Q = T[c + OFF(o->f[f][3])];
+            if ((M <= N) && (M <= P) &&
+                ((2.0f * (P - M) * (P - M) + (N - M) * (N - M)) <= hs2) &&
+                ((2.0f * (N - M) * (N - M) + (P - M) * (P - M)) <= hs2) &&
+                ((N - M) * (N - M) + (P - M) * (P - M) + (N - M) * (P - M) >= 0.5f * hs2)) {
+                Tijk = N + P - M + sqrtf(hs2 - (N - M) * (N - M) - (P - M) * (P - M));
+                if (Tijk < lowest) lowest = Tijk;
+            }
+            if ((N <= Q) && (P <= Q) &&
+                ((Q - N) * (Q - N) + (Q - P) * (Q - P) + (Q - N) * (Q - P) <= 0.5f * hs2)) {
+                Tijk = Q + sqrtf(hs2 - (Q - N) * (Q - N) - (Q - P) * (Q - P));
+                if (Tijk < lowest) lowest = Tijk;
+            }
+            if ((N - M >= 0) && (N - M <= Q - N) &&
+                (2 * (Q - N) * (Q - N) + (N - M) * (N - M) <= hs2)) {
+                Tijk = Q + sqrtf(hs2 - (Q - N) * (Q - N) - (N - M) * (N - M));
+                if (Tijk < lowest) lowest = Tijk;
+            }
+            if ((P - M >= 0) && (P - M <= Q - P) &&
+                (2 * (Q - P) * (Q - P) + (P - M) * (P - M) <= hs2)) {
+                Tijk = Q + sqrtf(hs2 - (Q - P) * (Q - P) - (P - M) * (P - M));
+                if (Tijk < lowest) lowest = Tijk;
+            }
+        }
+    }
+#undef OFF
+    return lowest;
+}
+
+void or_podvin(const float *slowness, int nx, int ny, int nz, float dx, float dy, float dz,
+               float sx, float sy, float sz, float *Tout)
+{
+    const int nxx = nx + 2, nyy = ny + 2, nzz = nz + 2;
+    const long NZ = nzz, NXZ = (long)nxx * nzz, N = NXZ * nyy;
+    float *S = (float *)malloc(sizeof(float) * N), *T = (float *)malloc(sizeof(float) * N);
+    float *nT = (float *)malloc(sizeof(float) * N);
+    unsigned char *K = (unsigned char *)calloc(N, 1), *nK = (unsigned char *)calloc(N, 1);
+    /* expand_model (classic.cpp:8-60): replicate the edge cells outwards */
+    for (int k = 0; k < nyy; k++)
+        for (int j = 0; j < nxx; j++)
+            for (int i = 0; i < nzz; i++)
+                S[i + j * NZ + k * NXZ] = slowness[imin(imax(i - 1, 0), nz - 1) +
+                                                   (long)imin(imax(j - 1, 0), nx - 1) * nz +
+                                                   (long)imin(imax(k - 1, 0), ny - 1) * nx * nz];
+    /* seeding (classic.cpp:186-246) */
+    const int sidx = (int)(sx / dx) + 1, sidy = (int)(sy / dy) + 1, sidz = (int)(sz / dz) + 1;
+    const long sId = sidz + sidx * NZ + sidy * NXZ;
+    for (long q = 0; q < N; q++) T[q] = nT[q] = 1e6f;
+    {
+        const float gx = floorf(sx / dx) * dx, gy = floorf(sy / dy) * dy, gz = floorf(sz / dz) * dz;
+        const float ex = sx - gx, ey = sy - gy, ez = sz - gz;
+        const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+        T[sId] = dist * S[sId];
+        nT[sId] = T[sId];
+    }
+    for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+            for (int e = -1; e <= 1; e++)
+                if (a || b || e) K[sId + a + b * NZ + e * NXZ] = 1;
+    /* iteration count: the largest source-to-corner distance in nodes (:248-273) */
+    int nIt = 0;
+    {
+        const float fx = (float)sidx, fy = (float)sidy, fz = (float)sidz;
+        const float gx = (float)(nxx - sidx), gy = (float)(nyy - sidy), gz = (float)(nzz - sidz);
+        const float c[8][3] = {{fx, fy, fz}, {gx, fy, fz}, {fx, gy, fz}, {fx, fy, gz},
+                               {fx, gy, gz}, {gx, fy, gz}, {gx, gy, fz}, {gx, gy, gz}};
+        for (int q = 0; q < 8; q++) {
+            const int aux = (int)sqrtf(c[q][0] * c[q][0] + c[q][1] * c[q][1] + c[q][2] * c[q][2]);
+            if (aux > nIt) nIt = aux;
+        }
+    }
+    const float h = dx, sqrt2 = sqrtf(2.0f), sqrt3 = sqrtf(3.0f);
+    (void)dy; (void)dz;
+    for (int it = 0; it < nIt; it++) {
+        for (int k = 1; k < nyy - 1; k++)
+            for (int j = 1; j < nxx - 1; j++)
+                for (int i = 1; i < nzz - 1; i++) {
+                    const long c = i + j * NZ + k * NXZ;
+                    if (!K[c]) continue;
+                    const float lowest = pod_update(T, S, c, NZ, NXZ, h, sqrt2, sqrt3);
+                    if (lowest == T[c]) K[c] = 0;
+                    nT[c] = lowest;
+                }
+        memset(nK, 0, N);
+        for (int k = 1; k < nyy - 1; k++)
+            for (int j = 1; j < nxx - 1; j++)
+                for (int i = 1; i < nzz - 1; i++) {
+                    const long c = i + j * NZ + k * NXZ;
+                    if (!K[c]) continue;
+                    for (int a = -1; a <= 1; a++)
+                        for (int b = -1; b <= 1; b++)
+                            for (int e = -1; e <= 1; e++)
+                                if (a || b || e) nK[c + a + b * NZ + e * NXZ] = 1;
+                }
+        memcpy(T, nT, sizeof(float) * N);
+        memcpy(K, nK, N);
+    }
+    /* reduce_model (:62-72) */
+    for (int k = 0; k < ny; k++)
+        for (int j = 0; j < nx; j++)
+            for (int i = 0; i < nz; i++)
+                Tout[i + (long)j * nz + (long)k * nx * nz] = T[(i + 1) + (j + 1) * NZ + (k + 1) * NXZ];
+    free(S); free(T); free(nT); free(K); free(nK);
+}

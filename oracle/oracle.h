@@ -85,6 +85,16 @@ void or_fsm_batch(const float *slowness, int nx, int ny, int nz,
                   const float *rx, const float *ry, const float *rz, int nr,
                   float *picks, int nthreads);
 
+/* ---- eikonal: Podvin & Lecomte (classic.cpp) ---- */
+/* Classic::eikonal_equation (classic.cpp:184-1842) for one shot: pad by one
+ * with replication (expand_model :8-60), seed the source node and activate
+ * its 26 neighbours, nItEikonal (:248-273) Jacobi iterations of the 1D head
+ * wave, edge and corner diffraction, 2D and 3D Huygens operators over the
+ * active cells, crop (reduce_model :62-72).  h = dx only, as the
+ * reference.  Output (nz,nx,ny), z fastest. */
+void or_podvin(const float *slowness, int nx, int ny, int nz, float dx, float dy, float dz,
+               float sx, float sy, float sz, float *T);
+
 /* ---- tomography: least squares (least_squares.cpp) ---- */
 
 /* expand_fdm (least_squares.cpp:11-67): pad T by one with replication */

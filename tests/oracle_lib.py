@@ -52,6 +52,7 @@ def oracle():
             "or_trilinear": (F, [F] * 17),
             "or_picks": (None, [fp, I, I, I, F, F, F, fp, fp, fp, I, fp]),
             "or_fsm": (None, [fp, I, I, I, F, F, F, F, F, F, fp]),
+            "or_podvin": (None, [fp, I, I, I, F, F, F, F, F, F, fp]),
             "or_fsm_batch": (None, [fp, I, I, I, F, F, F, fp, I, fp, fp, fp, I, fp, I]),
             "or_expand_fdm": (None, [fp, I, I, I, fp]),
             "or_ray_trace_shot": (L, [fp, I, I, I, F, F, F, I, I, I, F, F, F, F, F, F,

@@ -50,11 +50,29 @@ def test_regular_geometry_bit_exact(fx):
     assert np.array_equal(np.stack([x, y, z], 1), f["nodes_xyz"])
 
 
-def test_podvin_fixture_is_close_to_fsm():
-    """Podvin & Lecomte (classic.cpp) has no restatement here; its fixture is
-    kept as the reference's cross-check and agrees with the FSM to within
-    discretisation error."""
-    f = load("ref_params.npz")
+@pytest.mark.parametrize("fx", ["ref_params.npz", "ref_params_recip.npz"])
+def test_podvin_picks_bit_exact(fx):
+    """The Podvin & Lecomte restatement reproduces the picks the reference's
+    own classic.cpp wrote for every shot of the parameter file, and stays
+    within discretisation error of the FSM."""
+    f = load(fx)
+    if "pod_picks" not in f:
+        pytest.skip("fixture without Podvin picks")
+    vel = f["vel"]
+    ny, nx, nz = vel.shape
+    h = 25.0
+    recip = "reciprocity = True" in str(f["params_text"])
+    shots, nodes = f["shots_xyz"], f["nodes_xyz"]
+    if recip:
+        shots, nodes = nodes, shots
+    o = orc.oracle()
+    s = np.ascontiguousarray((1.0 / vel).astype(np.float32))
+    for i, src in enumerate(shots):
+        T = np.zeros(vel.shape, np.float32)
+        o.or_podvin(P(s), nx, ny, nz, h, h, h, *(float(v) for v in src), P(T))
+        if i == 0:
+            assert np.array_equal(T, f["pod_T_shot1"])
+        assert np.array_equal(orc.picks(T, (h, h, h), nodes), f["pod_picks"][i]), i
     assert np.abs(f["pod_picks"] - f["fsm_picks"]).max() < 0.05
 
 
