@@ -138,6 +138,37 @@ def fsm_leg(ctx, src, rcv, cpu_value):
     return out
 
 
+def podvin_leg(ctx, case, src, rcv, with_cpu):
+    """The reference's Podvin & Lecomte eikonal (eikonal_type 0) on the
+    device, bit-identical to its CPU code, over the same shots; beside it
+    the reference's own classic.cpp build on one shot, one core."""
+    ctx.podvin_solve(src[:16], rcv)
+    t0 = time.perf_counter()
+    _, _, ms = ctx.podvin_solve(src, rcv)
+    dt = time.perf_counter() - t0
+    out = {"value": len(src) / dt, "unit": "shot-solves/s", "ms_per_step": 1e3 * dt,
+           "device_ms": ms}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        from oracle_lib import P
+        if orc.ref_available():
+            vel, h = case["vel"], case["h"]
+            ny, nx, nz = vel.shape
+            s = np.ascontiguousarray((1.0 / vel).astype(np.float32))
+            T = np.zeros(s.shape, np.float32)
+            r = np.zeros(1, np.float32)
+            s0 = np.ascontiguousarray(src[:1])
+            t0 = time.perf_counter()
+            orc.ref().ref_eikonal(0, P(s), nx, ny, nz, h, h, h, P(s0), 1, P(r), P(r), P(r), 0,
+                                  P(T), None, None)
+            cpu = time.perf_counter() - t0
+            out["cpu_reference"] = {"value": 1.0 / cpu, "unit": "shot-solves/s", "cores": 1,
+                                    "kind": "reference", "sample": "1 shot (the first)"}
+            out["speedup_vs_cpu_reference_one_core"] = out["value"] * cpu
+    return out
+
+
 def tomography_leg(with_cpu):
     """One least-squares tomography iteration at the C2 configuration (16
     shots, 2,500 receivers, 201x201x101 at 12.5 m, inversion grid 41x41x21,
@@ -499,6 +530,8 @@ def main():
         cpu_v = out["cpu_baseline"]["value"] if "cpu_baseline" in out else None
         out["accurate_fsm_gpu"] = fsm_leg(ctx, src, rcv, cpu_v)
         log(f"accurate_fsm_gpu: {out['accurate_fsm_gpu']}")
+        out["podvin_gpu"] = podvin_leg(ctx, case, src, rcv, not a.no_cpu)
+        log(f"podvin_gpu: {out['podvin_gpu']}")
     ctx.close()
     if not a.no_extra:
         out["tomography_iteration"] = tomography_leg(not a.no_cpu)

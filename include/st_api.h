@@ -225,6 +225,15 @@ int st_nn_gemm(st_ctx *ctx, const float *A, const float *B, float *C, int M,
  * Replaces dampers (acoustic.cpp:100-137). */
 int st_acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
 
+/* The reference's Podvin & Lecomte (1991) eikonal (classic.cpp:184-1842,
+ * Classic::eikonal_equation with write_first_arrival :1850-1884;
+ * eikonal_type 0) on the device for all shots of the call, bit-identical to
+ * the CPU code.  Same arguments as st_fsm_solve; batch = shots per device
+ * pass (0 = 16).  Replaces Classic::eikonal_equation per shot. */
+int st_podvin_solve(st_ctx *ctx, const float *src_xyz, int nshot,
+                    const float *rcv_xyz, int nrcv, float *picks, float *T_out,
+                    int batch, double *device_ms);
+
 /* Ricker wavelet on the host: out[t] for t*dt - delay, peak frequency
  * parameter fmax.  Replaces rickerGeneration (acoustic.cpp:81-98). */
 int st_ricker(float delay, float fmax, float dt, int nt, float *out);

@@ -62,6 +62,9 @@ SIGNATURES = [
     ("st_fsm_solve", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
                                     ctypes.c_int, c_float_p, c_float_p, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]),
+    ("st_podvin_solve", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                       ctypes.c_int, c_float_p, c_float_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]),
     ("st_picks_volume", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, c_float_p, ctypes.c_int, c_float_p]),

@@ -129,6 +129,20 @@ class Context:
                                           ctypes.byref(ms)))
         return picks, T, ms.value
 
+    def podvin_solve(self, src, rcv, return_T=False, batch=0):
+        """The reference's Podvin & Lecomte eikonal on the device, bit-identical
+        to its CPU code.  Returns (picks[nshot, nrcv], T or None, device ms)."""
+        src = _f32(src, (-1, 3))
+        rcv = _f32(rcv, (-1, 3)) if rcv is not None else np.zeros((0, 3), np.float32)
+        ns, nr = src.shape[0], rcv.shape[0]
+        picks = np.empty((ns, nr), np.float32)
+        T = np.empty((ns,) + self.shape, np.float32) if return_T else None
+        ms = ctypes.c_double()
+        self._check(self.lib.st_podvin_solve(self.h, _ptr(src), ns, _ptr(rcv) if nr else None, nr,
+                                             _ptr(picks) if nr else None, _ptr(T), int(batch),
+                                             ctypes.byref(ms)))
+        return picks, T, ms.value
+
     def picks_volume(self, T, spacing, rcv):
         T = _f32(T)
         ny, nx, nz = T.shape

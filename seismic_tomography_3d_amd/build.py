@@ -34,15 +34,15 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 # files whose results must be bit-identical to the CPU formulas
-NO_CONTRACT = {"picks.hip", "tomo.hip", "fsm.hip", "wave.hip"}
+NO_CONTRACT = {"picks.hip", "tomo.hip", "fsm.hip", "wave.hip", "podvin.hip"}
 # files whose kernels never form or receive a NaN (inputs are checked or
 # sanitised on the bits); lets min/max skip quieting their operands
 NO_NANS = {"fim.hip"}
 
 HIP_SOURCES = ["api.hip", "fim.hip", "picks.hip", "tomo.hip", "surrogate.hip", "nn.hip", "fsm.hip",
-               "wave.hip"]
+               "wave.hip", "podvin.hip"]
 HOST_SOURCES = ["file_manager.cpp", "geometry.cpp", "device_eikonal.cpp", "block_fim.cpp",
-                "accurate_fsm.cpp", "least_squares.cpp"]
+                "accurate_fsm.cpp", "classic.cpp", "least_squares.cpp"]
 
 
 def _run(cmd):

@@ -31,6 +31,13 @@ float surrogate_cold_radius(st_ctx *ctx);
 void tomo_release(st_ctx *ctx);
 void surrogate_release(st_ctx *ctx);
 void wave_release(st_ctx *ctx);
+size_t podvin_bytes_per_shot(const st_ctx *ctx);
+double podvin_batch(st_ctx *ctx, const float *dsrc, const float *hsrc, int nshot, const float *S,
+                    float *T, int *stamp, int *list, float *newv, int *count, int *nit_dev,
+                    DevBuf &temp);
+void podvin_picks(st_ctx *ctx, const float *T, int nshot, const float *drcv, int nrcv,
+                  float *dpicks);
+void podvin_crop_volumes(st_ctx *ctx, const float *T, float *dout, int nshot);
 void acoustic_dampers(float *d1, float *d2, float *d3, float factor, int nb);
 void acoustic_ricker(float delay, float fmax, float dt, int nt, float *out);
 void wave_set_model(st_ctx *ctx, const float *vel, int nx, int ny, int nz, float dx, float dy,
@@ -147,7 +154,9 @@ int st_ctx_destroy(st_ctx *ctx) {
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (auto *b : {&ctx->slow_zxy, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
                     &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->wstats, &ctx->fsm_S,
-                    &ctx->fsm_sid, &ctx->src, &ctx->rcv,
+                    &ctx->fsm_sid, &ctx->pod_T, &ctx->pod_stamp, &ctx->pod_list,
+                    &ctx->pod_newv, &ctx->pod_counts, &ctx->pod_nit, &ctx->pod_temp,
+                    &ctx->src, &ctx->rcv,
                     &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred})
         b->release();
     for (auto &e : ctx->ev)
@@ -406,6 +415,77 @@ int st_fsm_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_
                           ctx->picks.as<float>() + (long long)b0 * nrcv);
             if (T_out) {
                 fsm_crop_volumes(ctx, ctx->scratch.as<float>(), nb);
+                ST_HIP(hipMemcpyAsync(T_out + (long long)b0 * npts, ctx->scratch.ptr,
+                                      sizeof(float) * npts * nb, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+                ST_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        }
+        if (nrcv)
+            ST_HIP(hipMemcpyAsync(picks, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+        if (device_ms) *device_ms = ms;
+    });
+}
+
+int st_podvin_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz, int nrcv,
+                    float *picks, float *T_out, int batch, double *device_ms) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(nshot >= 0 && nrcv >= 0, ST_ERR_ARG, "negative count");
+        ST_REQUIRE(nshot == 0 || src_xyz, ST_ERR_ARG, "null sources");
+        ST_REQUIRE(nrcv == 0 || (rcv_xyz && picks), ST_ERR_ARG, "null receivers or picks");
+        if (device_ms) *device_ms = 0.0;
+        if (nshot == 0) return;
+        check_points(src_xyz, nshot, ctx, "source");
+        check_points(rcv_xyz, nrcv, ctx, "receiver");
+        set_device(ctx);
+        const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+        const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+        if (!ctx->fsm_valid) {   // the same one-cell replicated padding as the sweeping solver
+            ctx->fsm_S.reserve(sizeof(float) * nB);
+            fsm_pad(ctx, ctx->fsm_S.as<float>());
+            ctx->fsm_valid = true;
+        }
+        int b = batch > 0 ? std::min(batch, nshot) : std::min(nshot, 16);
+        // list items and counts are 32-bit: shots x padded cells below 2^31
+        while (b > 1 && (long long)b * nB >= (1LL << 31)) b /= 2;
+        size_t free_b = 0, total_b = 0;
+        ST_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t per_shot = podvin_bytes_per_shot(ctx) + (T_out ? sizeof(float) * npts : 0);
+        const size_t have = free_b + ctx->pod_T.bytes + ctx->pod_stamp.bytes +
+                            ctx->pod_list.bytes + ctx->pod_newv.bytes + ctx->scratch.bytes;
+        while (b > 1 && per_shot * (size_t)b > have / 2) b /= 2;
+        ctx->pod_T.reserve(sizeof(float) * nB * b);
+        ctx->pod_stamp.reserve(sizeof(int) * nB * b);
+        ctx->pod_list.reserve(sizeof(int) * nB * b);
+        ctx->pod_newv.reserve(sizeof(float) * nB * b);
+        ctx->pod_counts.reserve(sizeof(int) * 64);
+        ctx->pod_nit.reserve(sizeof(int) * b);
+        if (T_out) ctx->scratch.reserve(sizeof(float) * npts * b);
+        ctx->src.reserve(sizeof(float) * 3 * nshot);
+        ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
+        ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
+        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
+                              hipMemcpyHostToDevice, ctx->stream));
+        if (nrcv)
+            ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
+                                  hipMemcpyHostToDevice, ctx->stream));
+        double ms = 0.0;
+        for (int b0 = 0; b0 < nshot; b0 += b) {
+            const int nb = std::min(b, nshot - b0);
+            ms += podvin_batch(ctx, ctx->src.as<float>() + 3LL * b0, src_xyz + 3LL * b0, nb,
+                               ctx->fsm_S.as<float>(), ctx->pod_T.as<float>(),
+                               ctx->pod_stamp.as<int>(), ctx->pod_list.as<int>(),
+                               ctx->pod_newv.as<float>(), ctx->pod_counts.as<int>(),
+                               ctx->pod_nit.as<int>(), ctx->pod_temp);
+            if (nrcv)
+                podvin_picks(ctx, ctx->pod_T.as<float>(), nb, ctx->rcv.as<float>(), nrcv,
+                             ctx->picks.as<float>() + (long long)b0 * nrcv);
+            if (T_out) {
+                podvin_crop_volumes(ctx, ctx->pod_T.as<float>(), ctx->scratch.as<float>(), nb);
                 ST_HIP(hipMemcpyAsync(T_out + (long long)b0 * npts, ctx->scratch.ptr,
                                       sizeof(float) * npts * nb, hipMemcpyDeviceToHost,
                                       ctx->stream));

@@ -110,4 +110,16 @@ void launch_picks_pad(hipStream_t st, const float *T, int NS, int nshot, const f
     ST_HIP(hipGetLastError());
 }
 
+void launch_picks_padvol(hipStream_t st, const float *T, long long vol, int nshot,
+                         const float *rcv, int nr, float *out, int nx, int ny, int nz, float dx,
+                         float dy, float dz) {
+    const long long n = (long long)nshot * nr;
+    if (n == 0) return;
+    // one padded volume per shot, shot stride vol
+    hipLaunchKernelGGL(picks_kernel<PadLayout>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       T, vol, nshot, rcv, nr, out, nx, ny, nz, dx, dy, dz,
+                       PadLayout{nx + 2, nz + 2, 1});
+    ST_HIP(hipGetLastError());
+}
+
 }  // namespace st

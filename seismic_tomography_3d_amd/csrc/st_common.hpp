@@ -128,6 +128,7 @@ struct st_ctx {
     st::DevBuf counters;               // list counts, stats
     st::DevBuf wstats;                 // per-wave FIM statistics slots
     st::DevBuf fsm_S, fsm_sid;         // sweeping solver: padded slowness, source nodes
+    st::DevBuf pod_T, pod_stamp, pod_list, pod_newv, pod_counts, pod_nit, pod_temp;   // Podvin
     bool fsm_valid = false;            // fsm_S matches the model
     st::DevBuf src, rcv, picks, scratch;
     st::DevBuf keys, kmin, judge, pred;   // warm-start state

@@ -1,6 +1,6 @@
 // eikonal_main.cpp -- eikonal driver (src/eikonal/eikonal_main.cpp:10-50):
-// the parameter file's eikonal_type picks the solver (1 block FIM, 2
-// Accurate_FSM; both on the device).  Usage: st_eikonal parameters.txt
+// the parameter file's eikonal_type picks the solver (0 Podvin & Lecomte,
+// 1 block FIM, 2 Accurate_FSM; all on the device).  Usage: st_eikonal parameters.txt
 // [--batched] -- with --batched all shots are solved in device batches.
 #include <chrono>
 #include <cstring>
@@ -10,6 +10,7 @@
 
 #include "accurate_fsm.hpp"
 #include "block_fim.hpp"
+#include "classic.hpp"
 #include "file_manager.hpp"
 
 int main(int argc, char **argv) {
@@ -21,13 +22,15 @@ int main(int argc, char **argv) {
     try {
         const int type = std::stoi(st_host::catch_parameter("eikonal_type", argv[1]));
         std::unique_ptr<st_host::DeviceEikonal> e;
-        if (type == 1)
+        if (type == 0)
+            e.reset(new st_host::Classic());
+        else if (type == 1)
             e.reset(new st_host::Block_FIM());
         else if (type == 2)
             e.reset(new st_host::Accurate_FSM());
         else
             throw std::invalid_argument("eikonal_type " + std::to_string(type) +
-                                        ": only 1 (block FIM) and 2 (Accurate_FSM) run here");
+                                        ": 0 (Podvin), 1 (block FIM) or 2 (Accurate_FSM)");
         e->parameters = argv[1];
         e->set_parameters();
         e->prepare_volumes();
@@ -41,7 +44,9 @@ int main(int argc, char **argv) {
                 std::cout << "rounds " << st.rounds << ", block visits " << st.visits << "\n";
             } else {
                 const double ms =
-                    static_cast<st_host::Accurate_FSM *>(e.get())->solve_all_shots(picks.data());
+                    type == 2
+                        ? static_cast<st_host::Accurate_FSM *>(e.get())->solve_all_shots(picks.data())
+                        : static_cast<st_host::Classic *>(e.get())->solve_all_shots(picks.data());
                 std::cout << "device time " << ms << " ms\n";
             }
             for (int i = 0; i < e->total_shots; i++) {

@@ -1,10 +1,11 @@
 """The host mirror driven exactly like the reference: a parameter file in,
 the reference's output files out (bin/st_eikonal over the C ABI).
 
-eikonal_type = 2 (Accurate_FSM) must reproduce, bit for bit, the files the
-reference's own CPU build wrote for the same parameter file (the golden
-fixtures hold them: picks of every shot and the travel-time volume of shot
-1, with reciprocity off and on), per shot and batched.  eikonal_type = 1
+eikonal_type = 2 (Accurate_FSM) and 0 (Podvin & Lecomte) must reproduce,
+bit for bit, the files the reference's own CPU build wrote for the same
+parameter file (the golden fixtures hold them: picks of every shot and the
+travel-time volume of shot 1, with reciprocity off and on), per shot and
+batched.  eikonal_type = 1
 (block FIM) is held to the FP64 solution of its discretisation (1e-4 s).
 """
 import os
@@ -44,19 +45,22 @@ def _run(pf, *extra):
 @pytest.mark.skipif(not os.path.exists(BIN), reason="st_eikonal not built")
 @pytest.mark.parametrize("fx", ["ref_params.npz", "ref_params_recip.npz"])
 @pytest.mark.parametrize("batched", [False, True])
-def test_fsm_driver_reproduces_reference_files(fx, batched):
-    f, d, pf = _setup(fx, 2)
+@pytest.mark.parametrize("etype,tag", [(2, "fsm"), (0, "pod")])
+def test_cpu_solver_drivers_reproduce_reference_files(fx, batched, etype, tag):
+    """eikonal_type 2 (Accurate_FSM) and 0 (Podvin & Lecomte) write the
+    files the reference's own build wrote, bit for bit."""
+    f, d, pf = _setup(fx, etype)
     _run(pf, *(["--batched"] if batched else []))
-    picks = f["fsm_picks"]
+    picks = f[f"{tag}_picks"]
     ns, nn = picks.shape
     for s in range(ns):
-        got = np.fromfile(os.path.join(d, f"fsm_times_nr{nn}_shot_{s + 1}.bin"), np.float32)
+        got = np.fromfile(os.path.join(d, f"{tag}_times_nr{nn}_shot_{s + 1}.bin"), np.float32)
         assert np.array_equal(got, picks[s]), (fx, s)
     if not batched:
-        ny, nx, nz = f["fsm_T_shot1"].shape
-        T = np.fromfile(os.path.join(d, f"fsm_eikonal_nz{nz}_nx{nx}_ny{ny}_shot_1.bin"),
+        ny, nx, nz = f[f"{tag}_T_shot1"].shape
+        T = np.fromfile(os.path.join(d, f"{tag}_eikonal_nz{nz}_nx{nx}_ny{ny}_shot_1.bin"),
                         np.float32).reshape(ny, nx, nz)
-        assert np.array_equal(T, f["fsm_T_shot1"])
+        assert np.array_equal(T, f[f"{tag}_T_shot1"])
 
 
 @pytest.mark.skipif(not os.path.exists(BIN), reason="st_eikonal not built")
