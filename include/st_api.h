@@ -215,6 +215,19 @@ int st_surrogate_predict(st_ctx *ctx, const float *src_xyz, int nshot, float *y)
 int st_nn_gemm(st_ctx *ctx, const float *A, const float *B, float *C, int M,
                int N, int K);
 
+/* One least-squares step after st_forward: the right-hand side, Tikhonov
+ * rows and A^T are assembled, then the CGLS iterations and the model
+ * update (with Gaussian smoothing when smooth != 0) run as one captured
+ * HIP graph (use_graph != 0) or eagerly; the same kernels in the same
+ * order, so both give the same result.  Replaces Least_squares::
+ * optimization followed by model_update (least_squares.cpp:321-570);
+ * xdm (n_model), dm and velocity (nx*ny*nz) are optional outputs as for
+ * st_optimize and st_model_update. */
+int st_optimize_update(st_ctx *ctx, const float *dobs, const float *dcal, int n,
+                       int tk_order, float lambda, int iters, int smooth,
+                       int samples, float stdv, int use_graph, float *xdm,
+                       float *dm, float *velocity);
+
 /* ---- data synthesis --------------------------------------------------
  * The acoustic modelling of src/utils/wave (8th-order Laplacian, Cerjan
  * absorbing layers) and the first-break auto-picker of

@@ -85,6 +85,10 @@ SIGNATURES = [
                                    ctypes.c_int, ctypes.c_float, ctypes.c_int, c_float_p]),
     ("st_cgls_csr", ctypes.c_int, [ctypes.c_void_p, c_ll_p, c_int_p, c_float_p, ctypes.c_int,
                                    ctypes.c_int, c_float_p, ctypes.c_int, c_float_p]),
+    ("st_optimize_update", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                          ctypes.c_int, c_float_p, c_float_p, c_float_p]),
     ("st_model_update", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_float, c_float_p, c_float_p]),
     ("st_surrogate_init", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint]),

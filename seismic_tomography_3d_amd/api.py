@@ -279,6 +279,25 @@ class Context:
                                          len(rowptr) - 1, M, _ptr(b), iters, _ptr(x)))
         return x
 
+    def optimize_update(self, dobs, dcal, tk_order, lam, iters, smooth=True, samples=5,
+                        stdv=2.0, graph=True, fetch=True):
+        """CGLS and the model update in one step (one captured HIP graph when
+        graph=True).  Returns (x, dm, velocity), or (None, None, None) when
+        fetch=False (the updated model stays on the device)."""
+        dobs = _f32(dobs, (-1,))
+        dcal = _f32(dcal, (-1,))
+        nxt, nyt, nzt = self.tomo_dims()
+        x = dm = vel = None
+        if fetch:
+            x = np.empty(nxt * nyt * nzt, np.float32)
+            dm = np.empty(self.shape, np.float32)
+            vel = np.empty(self.shape, np.float32)
+        self._check(self.lib.st_optimize_update(self.h, _ptr(dobs), _ptr(dcal), dobs.shape[0],
+                                                int(tk_order), lam, int(iters), int(bool(smooth)),
+                                                int(samples), stdv, int(bool(graph)), _ptr(x),
+                                                _ptr(dm), _ptr(vel)))
+        return x, dm, vel
+
     def model_update(self, xdm=None, smooth=False, samples=5, stdv=2.0):
         dm = np.empty(self.shape, np.float32)
         vel = np.empty(self.shape, np.float32)

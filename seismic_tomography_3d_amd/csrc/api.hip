@@ -63,6 +63,9 @@ void tomo_load_csr(st_ctx *ctx, const long long *rowptr, const int *cols, const 
                    long long nnz);
 void tomo_set_rhs(st_ctx *ctx, const float *dobs, const float *dcal, int n, long long Ntot);
 void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters);
+void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda);
+void tomo_solve_update(st_ctx *ctx, int iters, int smooth, int samples, float stdv, bool graph,
+                       float *x_out, float *dm_out, float *vel_out);
 const float *tomo_x(st_ctx *ctx);
 double tomo_residual(st_ctx *ctx, const float *dobs, const float *dcal, int n);
 void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, float stdv,
@@ -742,6 +745,31 @@ int st_cgls_csr(st_ctx *ctx, const long long *rowptr, const int *cols, const flo
         ST_HIP(hipMemcpyAsync(x, tomo_x(ctx), sizeof(float) * M, hipMemcpyDeviceToHost,
                               ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int st_optimize_update(st_ctx *ctx, const float *dobs, const float *dcal, int n, int tk_order,
+                       float lambda, int iters, int smooth, int samples, float stdv, int use_graph,
+                       float *xdm, float *dm, float *velocity) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        int nt[3], ms, mr;
+        float d[3];
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(tomo_grid(ctx, nt, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+        long long rows = 0, nnz = 0;
+        const long long *rp;
+        const int *c;
+        const float *v;
+        tomo_csr(ctx, &rows, &nnz, &rp, &c, &v);
+        ST_REQUIRE(rows > 0 && rows == n, ST_ERR_STATE,
+                   "st_optimize_update needs the G of st_forward with n = shots*receivers rows");
+        ST_REQUIRE(dobs && dcal && iters >= 0 && tk_order >= 0, ST_ERR_ARG, "bad arguments");
+        set_device(ctx);
+        const int n_model = nt[0] * nt[1] * nt[2];
+        tomo_set_rhs(ctx, dobs, dcal, n, rows + n_model - tk_order);
+        tomo_cgls_setup(ctx, n_model, tk_order, lambda);
+        tomo_solve_update(ctx, iters, smooth, samples, stdv, use_graph != 0, xdm, dm, velocity);
     });
 }
 

@@ -55,6 +55,9 @@ struct TomoState {
     bool have_x = false;
 
     DevBuf dm, upd, kern;
+    DevBuf gkern;                  // Gaussian weights read by the captured update
+    hipGraphExec_t gexec = nullptr;   // CGLS + model update, captured
+    std::vector<long long> gkey;      // what gexec was captured for
 };
 
 TomoState &tomo(st_ctx *ctx) {
@@ -68,8 +71,9 @@ void tomo_release(st_ctx *ctx) {
     for (DevBuf *b : {&t.rowptr, &t.cols, &t.vals, &t.runs, &t.nruns, &t.fdist, &t.rowlen,
                       &t.scanbuf, &t.err, &t.illum, &t.t_colptr, &t.t_rows, &t.t_vals,
                       &t.t_cursor, &t.t_keys, &t.t_idx0, &t.t_idx1, &t.t_sort, &t.b, &t.s, &t.q, &t.r, &t.p, &t.x, &t.dots, &t.scal, &t.dm,
-                      &t.upd, &t.kern})
+                      &t.upd, &t.kern, &t.gkern})
         b->release();
+    if (t.gexec) (void)hipGraphExecDestroy(t.gexec);
     delete &t;
     ctx->tomo = nullptr;
 }
@@ -488,8 +492,18 @@ __global__ void sub_vec(const float *a, const float *b, float *out, long long n,
 
 static unsigned nblk256(long long n) { return (unsigned)((n + 255) / 256); }
 
-// Assembles A = [G; lambda*L] in place after G, builds A^T, runs CGLS.
-void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
+// copies and fills as kernels (they may be captured into a graph)
+__global__ void copy_vec(float *__restrict__ dst, const float *__restrict__ src, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+__global__ void zero_vec(float *__restrict__ dst, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = 0.0f;
+}
+
+// Assembles A = [G; lambda*L] in place after G and builds A^T.
+void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda) {
     TomoState &t = tomo(ctx);
     const long long n_data = t.rows;
     ST_REQUIRE(order >= 0 && order < n_model, ST_ERR_ARG, "bad Tikhonov order");
@@ -567,6 +581,15 @@ void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
     t.x.reserve(sizeof(float) * n_model);
     t.dots.reserve(sizeof(double) * (kDotBlocks + 8));
     t.scal.reserve(sizeof(float) * 8);
+}
+
+// The CGLS iterations on the assembled system (least_squares.cpp:424-496):
+// kernels only (no copies or fills through the runtime), so the sequence
+// can be captured into a graph.
+void tomo_cgls_iterate(st_ctx *ctx, int iters) {
+    TomoState &t = tomo(ctx);
+    const long long N = t.N;
+    const int n_model = t.n_model;
     double *part = t.dots.as<double>();
     double *qTq = part + kDotBlocks, *rTr = qTq + 1, *rd = qTq + 2;
     float *alpha = t.scal.as<float>(), *beta = alpha + 1;
@@ -585,10 +608,13 @@ void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
     float *s = t.s.as<float>(), *q = t.q.as<float>(), *r = t.r.as<float>(), *p = t.p.as<float>(),
           *x = t.x.as<float>();
     // s = B (copied in by the caller into t.b), x = 0
-    ST_HIP(hipMemcpyAsync(s, t.b.ptr, sizeof(float) * N, hipMemcpyDeviceToDevice, ctx->stream));
-    ST_HIP(hipMemsetAsync(x, 0, sizeof(float) * n_model, ctx->stream));
+    hipLaunchKernelGGL(copy_vec, dim3(nblk256(N)), dim3(256), 0, ctx->stream, s,
+                       t.b.as<float>(), N);
+    hipLaunchKernelGGL(zero_vec, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, x,
+                       (long long)n_model);
     spmvt(s, r);
-    ST_HIP(hipMemcpyAsync(p, r, sizeof(float) * n_model, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(copy_vec, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, p, r,
+                       (long long)n_model);
     spmv(p, q);
     for (int it = 0; it < iters; it++) {
         dot(ctx, q, q, N, part, qTq);
@@ -606,8 +632,14 @@ void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
         spmv(p, q);
     }
     ST_HIP(hipGetLastError());
+}
+
+// Assembles the system and runs CGLS (eager).
+void tomo_cgls(st_ctx *ctx, int n_model, int order, float lambda, int iters) {
+    tomo_cgls_setup(ctx, n_model, order, lambda);
+    tomo_cgls_iterate(ctx, iters);
     ST_HIP(hipStreamSynchronize(ctx->stream));
-    t.have_x = true;
+    tomo(ctx).have_x = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -717,6 +749,121 @@ static void gaussian_weights(int samples, float stdv, std::vector<float> &kernel
                 }
         for (int i = 0; i < nK; i++) kernel[i] /= sum;
     }
+}
+
+// model_update kernels (least_squares.cpp:498-570): dm from the solution,
+// optional Gaussian smoothing with weights already on the device, slowness
+// update; the updated velocity lands in t.upd, dm in the returned buffer.
+// Kernels only, so the sequence can be captured into a graph.
+static float *update_kernels(st_ctx *ctx, TomoState &t, const float *dxdm, int smooth, int samples,
+                             const float *kern) {
+    const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
+    hipLaunchKernelGGL(dm_trilinear, dim3(nblk256(n)), dim3(256), 0, ctx->stream, dxdm,
+                       t.dm.as<float>(), ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz,
+                       t.nxt, t.nyt, t.nzt, t.dxt, t.dyt, t.dzt);
+    float *dm = t.dm.as<float>(), *tmp = t.upd.as<float>();
+    if (smooth) {
+        hipLaunchKernelGGL(gaussian3d, dim3(nblk256(n)), dim3(256), 0, ctx->stream, dm, tmp, kern,
+                           ctx->nx, ctx->ny, ctx->nz, samples);
+        std::swap(dm, tmp);
+    }
+    hipLaunchKernelGGL(apply_update, dim3(nblk256(n)), dim3(256), 0, ctx->stream,
+                       ctx->slow_zxy.as<float>(), dm, tmp, ctx->nx, ctx->ny, ctx->nz);
+    ST_HIP(hipGetLastError());
+    return dm;
+}
+
+// CGLS and the model update as one captured graph (or eagerly): the system
+// must be assembled (tomo_cgls_setup) and the right-hand side set.  The
+// graph is captured on every call (sizes and buffers follow the current
+// matrix) and the instantiated executable updated in place when its shape
+// allows, re-instantiated otherwise.
+void tomo_solve_update(st_ctx *ctx, int iters, int smooth, int samples, float stdv, bool graph,
+                       float *x_out, float *dm_out, float *vel_out) {
+    TomoState &t = tomo(ctx);
+    const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
+    t.dm.reserve(sizeof(float) * n);
+    t.upd.reserve(sizeof(float) * n);
+    if (smooth) {
+        ST_REQUIRE(samples >= 1 && (samples & 1), ST_ERR_ARG, "filter_samples must be odd");
+        std::vector<float> kern;
+        gaussian_weights(samples, stdv, kern);
+        t.gkern.reserve(sizeof(float) * kern.size());
+        ST_HIP(hipMemcpyAsync(t.gkern.ptr, kern.data(), sizeof(float) * kern.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));   // before the host vector goes
+    }
+    const float *kern = t.gkern.as<float>();
+    float *dm = nullptr;
+    // every buffer address and size the captured kernels bake in: an equal
+    // key means the instantiated graph can be launched as it is
+    const std::vector<long long> key = {
+        (long long)t.b.ptr, (long long)t.s.ptr, (long long)t.q.ptr, (long long)t.r.ptr,
+        (long long)t.p.ptr, (long long)t.x.ptr, (long long)t.dots.ptr, (long long)t.scal.ptr,
+        (long long)t.rowptr.ptr, (long long)t.cols.ptr, (long long)t.vals.ptr,
+        (long long)t.t_colptr.ptr, (long long)t.t_rows.ptr, (long long)t.t_vals.ptr,
+        (long long)t.dm.ptr, (long long)t.upd.ptr, (long long)t.gkern.ptr,
+        (long long)ctx->slow_zxy.ptr, t.N, t.n_model, t.annz, iters, smooth, samples, ctx->nx,
+        ctx->ny, ctx->nz, t.nxt, t.nyt, t.nzt};
+    if (graph && t.gexec && key == t.gkey) {
+        ST_HIP(hipGraphLaunch(t.gexec, ctx->stream));
+        dm = smooth ? t.upd.as<float>() : t.dm.as<float>();
+    } else if (graph) {
+        hipGraph_t g = nullptr;
+        t.gkey.clear();
+        ST_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            tomo_cgls_iterate(ctx, iters);
+            dm = update_kernels(ctx, t, t.x.as<float>(), smooth, samples, kern);
+        } catch (...) {
+            // leave the stream usable: end the capture and drop its graph
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            (void)hipGetLastError();
+            throw;
+        }
+        ST_HIP(hipStreamEndCapture(ctx->stream, &g));
+        bool updated = false;
+        if (t.gexec) {
+            hipGraphExecUpdateResult res;
+            hipGraphNode_t bad = nullptr;
+            updated = hipGraphExecUpdate(t.gexec, g, &bad, &res) == hipSuccess &&
+                      res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(t.gexec);
+                t.gexec = nullptr;
+            }
+        }
+        if (!updated) {
+            const hipError_t e = hipGraphInstantiate(&t.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) t.gexec = nullptr;
+            ST_HIP(e);
+        } else {
+            (void)hipGraphDestroy(g);
+        }
+        t.gkey = key;
+        ST_HIP(hipGraphLaunch(t.gexec, ctx->stream));
+    } else {
+        tomo_cgls_iterate(ctx, iters);
+        dm = update_kernels(ctx, t, t.x.as<float>(), smooth, samples, kern);
+    }
+    t.have_x = true;
+    if (x_out)
+        ST_HIP(hipMemcpyAsync(x_out, t.x.ptr, sizeof(float) * t.n_model, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    if (dm_out)
+        ST_HIP(hipMemcpyAsync(dm_out, dm, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (vel_out) {
+        // the updated velocity is in the buffer that did not receive dm
+        const float *vel = dm == t.dm.as<float>() ? t.upd.as<float>() : t.dm.as<float>();
+        ST_HIP(hipMemcpyAsync(vel_out, vel, sizeof(float) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->blk_valid[0] = ctx->blk_valid[1] = false;  // block layouts re-packed on next solve
 }
 
 void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, float stdv,

@@ -1,7 +1,7 @@
 """Full-size configurations checked through size-independent properties.
 
-C2 (one tomography iteration at 201x201x101, h = 12.5 m) and C5 (401x401x201,
-h = 10 m, one 64-shot chunk) are too large for the FP64 oracle to finish in
+One tomography iteration at 201x201x101 (h = 12.5 m) and one 64-shot chunk
+of the 401x401x201 survey (h = 10 m) are too large for the FP64 oracle to finish in
 seconds, so the GPU results are held to properties that fix them at any size:
 
 * the discrete eikonal equation: every interior non-source cell satisfies
@@ -56,8 +56,8 @@ def godunov_residual(T, s, h, src):
     return float(r[1:-1, 1:-1, 1:-1].max())
 
 
-def test_c5_chunk_properties(ctx):
-    """C5: 401x401x201 at 10 m, one 64-shot chunk of the 32x32 surface
+def test_401x401x201_chunk_properties(ctx):
+    """401x401x201 at 10 m, one 64-shot chunk of the 32x32 surface
     shots (every 16th), 64x64 receivers."""
     nx, ny, nz, h = 401, 401, 201, 10.0
     vel = synth.layered_model(nx, ny, nz, h, n_layers=5, n_anom=3, seed=2)
@@ -70,7 +70,7 @@ def test_c5_chunk_properties(ctx):
     src = np.ascontiguousarray(src_all[::16])
     rcv = np.stack(synth.surface_grid(64, 0.02 * L, 0.98 * L, 100.0), 1)
     picks, _, st = ctx.fim_solve(src, rcv, batch=64)
-    print(f"C5 64 shots: solve {st['solve_ms']:.1f} ms, total {st['total_ms']:.1f} ms, "
+    print(f"401x401x201, 64 shots: solve {st['solve_ms']:.1f} ms, total {st['total_ms']:.1f} ms, "
           f"rounds {st['rounds']}, visits {st['visits']}")
     assert np.all(np.isfinite(picks)) and picks.min() >= 0.0 and picks.max() < 10.0
     two = np.ascontiguousarray(src[[5, 40]])
@@ -86,8 +86,8 @@ def test_c5_chunk_properties(ctx):
         assert res < 5e-5
 
 
-def test_c2_tomography_iteration(ctx):
-    """C2: 16 surface shots, 2,500 seafloor receivers, picks = true-model FIM
+def test_201x201x101_tomography_iteration(ctx):
+    """201x201x101 at 12.5 m: 16 surface shots, 2,500 seafloor receivers, picks = true-model FIM
     + N(0, 2 ms) (seed 3); inversion grid 41x41x21, order 2, lambda 1e-2,
     20 CGLS iterations."""
     c = synth.tomography_case()
@@ -106,7 +106,7 @@ def test_c2_tomography_iteration(ctx):
     assert rp.shape[0] == n_rays + 1 and rp[-1] == vals.shape[0]
     rowsum = np.add.reduceat(vals.astype(np.float64), rp[:-1]) if vals.size else np.zeros(n_rays)
     dist = np.linalg.norm(np.repeat(src, rcv.shape[0], 0) - np.tile(rcv, (src.shape[0], 1)), axis=1)
-    print(f"C2: {n_rays} rays, nnz {vals.shape[0]}, rowsum/dist in "
+    print(f"201x201x101: {n_rays} rays, nnz {vals.shape[0]}, rowsum/dist in "
           f"[{(rowsum / dist).min():.3f}, {(rowsum / dist).max():.3f}]")
     assert np.all(rowsum >= 0.98 * dist) and np.all(rowsum <= 1.5 * dist + 100.0)
     r0 = ctx.residual_norm(dobs, dcal)
@@ -114,5 +114,5 @@ def test_c2_tomography_iteration(ctx):
     ctx.model_update(None, smooth=True, samples=5, stdv=2.0)
     dcal1, _, _ = ctx.forward(src, rcv)
     r1 = ctx.residual_norm(dobs, dcal1)
-    print(f"C2 misfit {r0:.4f} -> {r1:.4f}")
+    print(f"201x201x101 misfit {r0:.4f} -> {r1:.4f}")
     assert r1 < r0

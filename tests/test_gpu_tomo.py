@@ -190,3 +190,58 @@ def test_tomography_iteration_reduces_misfit(ctx):
     # fall monotonically, not by a fixed factor
     assert all(b < a for a, b in zip(r, r[1:]))
     assert r[-1] < 0.98 * r[0]
+
+
+def _solve_update_case(ctx, vel, h, src, rcv, d):
+    """Resets the model, rebuilds G with a forward pass and returns the
+    system's right-hand side (same inputs every time)."""
+    ctx.set_model(1.0 / vel, h)
+    ctx.tomo_set_grid(d)
+    dcal, _, _ = ctx.forward(src, rcv)
+    rng = np.random.default_rng(11)
+    dobs = (dcal + rng.normal(0, 2e-3, dcal.shape)).astype(np.float32)
+    return dobs, dcal
+
+
+def test_optimize_update_graph_equals_eager(ctx):
+    """CGLS + model update captured as one HIP graph gives bit for bit what
+    the eager sequence (st_optimize then st_model_update) gives, on every
+    replay, with eager work between the replays; the graph is re-captured
+    and updated in place when the smoothing switch changes the node list."""
+    vel, h, src, rcv = _small_tomo()
+    d = 5 * h
+    args = (2, 1e-2, 12)
+    ref = {}
+    for smooth in (True, False):
+        dobs, dcal = _solve_update_case(ctx, vel, h, src, rcv, d)
+        x = ctx.optimize(dobs, dcal, *args)
+        dm, v = ctx.model_update(None, smooth=smooth, samples=5, stdv=2.0)
+        ref[smooth] = (x, dm, v, ctx.get_model())
+    for graph, smooth in [(False, True), (True, True), (True, True), (True, False), (False, False),
+                          (True, False), (True, True), ("nofetch", True), (True, True)]:
+        dobs, dcal = _solve_update_case(ctx, vel, h, src, rcv, d)
+        _ = ctx.residual_norm(dobs, dcal)       # eager work between replays
+        x0, dm0, v0, s0 = ref[smooth]
+        tag = f"graph={graph} smooth={smooth}"
+        if graph == "nofetch":
+            out = ctx.optimize_update(dobs, dcal, *args, smooth=smooth, samples=5, stdv=2.0,
+                                      graph=True, fetch=False)
+            assert out == (None, None, None)
+            assert np.array_equal(ctx.get_model(), s0), tag
+            continue
+        x, dm, v = ctx.optimize_update(dobs, dcal, *args, smooth=smooth, samples=5, stdv=2.0,
+                                       graph=graph)
+        assert np.array_equal(x, x0), tag
+        assert np.array_equal(dm, dm0), tag
+        assert np.array_equal(v, v0), tag
+        assert np.array_equal(ctx.get_model(), s0), tag
+
+
+def test_optimize_update_rejects_bad_state(ctx):
+    from seismic_tomography_3d_amd.api import StError
+    vel, h, src, rcv = _small_tomo()
+    dobs, dcal = _solve_update_case(ctx, vel, h, src, rcv, 5 * h)
+    with pytest.raises(StError):
+        ctx.optimize_update(dobs[:-1], dcal[:-1], 2, 1e-2, 5)
+    with pytest.raises(StError):
+        ctx.optimize_update(dobs, dcal, 2, 1e-2, 5, smooth=True, samples=4)
