@@ -50,6 +50,7 @@ struct TomoState {
     int n_model = 0, n_data = 0, order = 0, N = 0, M = 0;
     long long annz = 0;
     DevBuf t_colptr, t_rows, t_vals, t_cursor;
+    int n_long = 0;                // A^T columns reduced by a whole workgroup
     DevBuf t_keys, t_idx0, t_idx1, t_sort;  // column-sort staging (radix sort pairs)
     DevBuf b, s, q, r, p, x, dots, scal;
     bool have_x = false;
@@ -100,10 +101,18 @@ struct ZxyLay {
         return z + (long long)x * nz + (long long)y * nx * nz;
     }
 };
+// solver block layout for a power-of-two block edge 1 << lb and in-volume
+// (non-negative) coordinates: shifts and masks instead of the three integer
+// divisions of blk_addr, and 32-bit offsets (the caller checks that one
+// shot's volume holds fewer than 2^31 cells) -- six lookups per ray step
 struct BlkLay {
-    int B, nbx, nby;
-    __device__ long long operator()(int x, int y, int z) const {
-        return blk_addr(x, y, z, B, nbx, nby);
+    int lb, nbx, nby;
+    __device__ unsigned operator()(int x, int y, int z) const {
+        const unsigned m = (1u << lb) - 1u;
+        const unsigned b = (unsigned)(x >> lb) +
+                           (unsigned)nbx * ((unsigned)(y >> lb) + (unsigned)nby * (unsigned)(z >> lb));
+        return (b << (3 * lb)) + ((unsigned)x & m) + (((unsigned)y & m) << lb) +
+               (((unsigned)z & m) << (2 * lb));
     }
 };
 
@@ -360,7 +369,9 @@ static void trace_append(st_ctx *ctx, const float *T, long long vol, L lay, cons
 
 void tomo_trace_blk(st_ctx *ctx, const float *T, long long vol, int B, int nbx, int nby,
                     const float *dsrc, int nshot, const float *drcv, int nr, bool with_illum) {
-    trace_append(ctx, T, vol, BlkLay{B, nbx, nby}, dsrc, nshot, drcv, nr, with_illum);
+    ST_REQUIRE(B == 4 || B == 8, ST_ERR_ARG, "block edge must be 4 or 8");
+    ST_REQUIRE(vol < (1LL << 31), ST_ERR_CAPACITY, "traveltime volume too large for ray tracing");
+    trace_append(ctx, T, vol, BlkLay{B == 8 ? 3 : 2, nbx, nby}, dsrc, nshot, drcv, nr, with_illum);
 }
 
 // ---------------------------------------------------------------------------
@@ -380,11 +391,6 @@ __global__ void tikhonov_rows(long long *rowptr, int *cols, float *vals, long lo
         cols[o + k] = i + k;
         vals[o + k] = w[k];
     }
-}
-
-__global__ void col_count(const int *__restrict__ cols, long long nnz, int *__restrict__ cnt) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < nnz) atomicAdd(&cnt[cols[e]], 1);
 }
 
 __global__ void iota_int(int *__restrict__ v, long long n) {
@@ -428,6 +434,67 @@ __global__ void spmv_csr(const long long *__restrict__ rowptr, const int *__rest
     if (row < nrows && l == 0) y[row] = acc;
 }
 
+// Column pointers of A^T from the column-sorted keys: colptr[c] = the first
+// sorted position whose column is >= c (one write per c, no atomics).
+__global__ void col_bounds(const unsigned *__restrict__ keys, long long nnz, int ncols,
+                           long long *__restrict__ colptr) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nnz) return;
+    const int prev = i > 0 ? (int)keys[i - 1] : -1;
+    const int cur = i < nnz ? (int)keys[i] : ncols;
+    for (int c = prev + 1; c <= cur; c++) colptr[c] = i;
+}
+
+// A^T's columns longer than kLongCol entries (the cells every ray of a shot
+// crosses near its source run to thousands): listed once per system, each
+// then reduced by a whole workgroup instead of 16 lanes.
+constexpr int kLongCol = 256;
+
+__global__ void long_rows(const long long *__restrict__ rowptr, int nrows, int *__restrict__ count,
+                          int *__restrict__ list) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < nrows && rowptr[row + 1] - rowptr[row] > kLongCol) list[atomicAdd(count, 1)] = row;
+}
+
+// y = A x for a CSR matrix with a few long rows: blocks [0, nshort) give
+// 16 lanes to each row of at most kLongCol entries (as spmv_csr), the
+// blocks after them one workgroup to each listed long row (strided partial
+// sums, fixed LDS tree) -- both orders are fixed, so results repeat.
+__global__ __launch_bounds__(256) void spmv_split(const long long *__restrict__ rowptr,
+                                                  const int *__restrict__ idx,
+                                                  const float *__restrict__ val,
+                                                  const float *__restrict__ x, float *__restrict__ y,
+                                                  int nrows, int nshort,
+                                                  const int *__restrict__ long_list) {
+    if ((int)blockIdx.x < nshort) {
+        const long long row = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+        const int l = threadIdx.x & 15;
+        float acc = 0.f;
+        bool mine = false;
+        if (row < nrows) {
+            const long long b = rowptr[row], e = rowptr[row + 1];
+            mine = e - b <= kLongCol;
+            if (mine)
+                for (long long k = b + l; k < e; k += 16) acc += val[k] * x[idx[k]];
+        }
+        for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+        if (mine && l == 0) y[row] = acc;
+        return;
+    }
+    __shared__ float sh[256];
+    const int row = long_list[blockIdx.x - nshort];
+    const long long b = rowptr[row], e = rowptr[row + 1];
+    float acc = 0.f;
+    for (long long k = b + threadIdx.x; k < e; k += 256) acc += val[k] * x[idx[k]];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) y[row] = sh[0];
+}
+
 // FP64 dot products: per-block partials, then one ordered pass
 __global__ void dot_partial(const float *__restrict__ a, const float *__restrict__ b, long long n,
                             double *__restrict__ part) {
@@ -464,18 +531,37 @@ static void dot(st_ctx *ctx, const float *a, const float *b, long long n, double
     hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, ctx->stream, part, kDotBlocks, out);
 }
 
-// scal[0] = num / den (FP64 ratio, used as FP32 step)
-// (a zero denominator -- an exactly solved system -- gives a zero step
-// where the reference would produce NaN)
-__global__ void ratio(const double *num, const double *den, float *out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *out = (*den != 0.0) ? (float)(*num / *den) : 0.0f;
+// dot_final, then a CGLS step from the sum and a stored dot (one thread):
+// step = sum / other (new_is_num) or other / sum, the FP64 ratio used as an
+// FP32 step (a zero denominator -- an exactly solved system -- gives a zero
+// step where the reference would produce NaN)
+__global__ void dot_final_ratio(const double *__restrict__ part, int n, double *__restrict__ out,
+                                const double *__restrict__ other, int new_is_num,
+                                float *__restrict__ step) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += part[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = sh[0];
+        const double num = new_is_num ? sh[0] : *other, den = new_is_num ? *other : sh[0];
+        *step = (den != 0.0) ? (float)(num / den) : 0.0f;
+    }
 }
 
-// y = y + sign * alpha * x
-__global__ void axpy(float *__restrict__ y, const float *__restrict__ x, const float *alpha,
-                     float sign, long long n) {
+// y1 += sign1 * alpha * x1 (n1 entries) and y2 += sign2 * alpha * x2 (n2
+// entries) in one launch
+__global__ void axpy2(float *__restrict__ y1, const float *__restrict__ x1, long long n1,
+                      float sign1, float *__restrict__ y2, const float *__restrict__ x2,
+                      long long n2, float sign2, const float *alpha) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] += sign * (*alpha) * x[i];
+    if (i < n1) y1[i] += sign1 * (*alpha) * x1[i];
+    if (i < n2) y2[i] += sign2 * (*alpha) * x2[i];
 }
 
 // p = r + beta * p
@@ -536,21 +622,14 @@ void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda) {
                        t.rowptr.as<long long>(), t.cols.as<int>(), t.vals.as<float>(), n_data,
                        t.nnz, nL, order, t.kern.as<float>());
 
-    // A^T (CSC of A), rows ascending inside each column
+    // A^T (CSC of A), rows ascending inside each column: a stable radix sort
+    // of the entry indices by column (the bits a column index needs), column
+    // pointers from the sorted keys, then rows and values gathered in that
+    // order
     t.t_colptr.reserve(sizeof(long long) * (n_model + 1));
     t.t_cursor.reserve(sizeof(int) * (n_model + 1));
     t.t_rows.reserve(sizeof(int) * annz);
     t.t_vals.reserve(sizeof(float) * annz);
-    ST_HIP(hipMemsetAsync(t.t_cursor.ptr, 0, sizeof(int) * (n_model + 1), ctx->stream));
-    hipLaunchKernelGGL(col_count, dim3(nblk256(annz)), dim3(256), 0, ctx->stream, t.cols.as<int>(),
-                       annz, t.t_cursor.as<int>());
-    const long long tot = exclusive_scan(ctx, t.t_cursor.as<int>(), t.t_colptr.as<long long>(),
-                                         n_model, 0, t.scanbuf);
-    ST_REQUIRE(tot == annz, ST_ERR_HIP, "transpose count mismatch");
-    ST_HIP(hipMemcpyAsync(t.t_colptr.as<long long>() + n_model, &tot, sizeof(long long),
-                          hipMemcpyHostToDevice, ctx->stream));
-    // stable radix sort of the entry indices by column (the bits a column
-    // index needs), then gather rows and values in that order
     ST_REQUIRE(annz < (1LL << 31), ST_ERR_CAPACITY, "too many non-zeros for the transpose");
     t.t_keys.reserve(sizeof(unsigned) * annz);
     t.t_idx0.reserve(sizeof(int) * annz);
@@ -568,10 +647,22 @@ void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda) {
     ST_HIP(rocprim::radix_sort_pairs(t.t_sort.ptr, tmp, keys_in, t.t_keys.as<unsigned>(),
                                      t.t_idx0.as<int>(), t.t_idx1.as<int>(), (size_t)annz, 0u, bits,
                                      ctx->stream));
+    hipLaunchKernelGGL(col_bounds, dim3(nblk256(annz + 1)), dim3(256), 0, ctx->stream,
+                       t.t_keys.as<unsigned>(), annz, n_model, t.t_colptr.as<long long>());
     hipLaunchKernelGGL(col_gather, dim3(nblk256(annz)), dim3(256), 0, ctx->stream,
                        t.rowptr.as<long long>(), (int)N, t.vals.as<float>(), t.t_idx1.as<int>(),
                        annz, t.t_rows.as<int>(), t.t_vals.as<float>());
+    // A^T's long columns: t_cursor[0] counts them, t_cursor[1..] lists them
+    hipLaunchKernelGGL(zero_vec, dim3(1), dim3(64), 0, ctx->stream,
+                       reinterpret_cast<float *>(t.t_cursor.ptr), 1LL);
+    hipLaunchKernelGGL(long_rows, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream,
+                       t.t_colptr.as<long long>(), n_model, t.t_cursor.as<int>(),
+                       t.t_cursor.as<int>() + 1);
     ST_HIP(hipGetLastError());
+    int *hcount = static_cast<int *>(ctx->host_pinned) + 64;
+    ST_HIP(hipMemcpyAsync(hcount, t.t_cursor.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    t.n_long = *hcount;
 
     t.N = (int)N; t.M = n_model; t.annz = annz; t.n_model = n_model; t.order = order;
     t.s.reserve(sizeof(float) * N);
@@ -591,7 +682,9 @@ void tomo_cgls_iterate(st_ctx *ctx, int iters) {
     const long long N = t.N;
     const int n_model = t.n_model;
     double *part = t.dots.as<double>();
-    double *qTq = part + kDotBlocks, *rTr = qTq + 1, *rd = qTq + 2;
+    // r.r is kept from the end of one iteration for the next (the reference
+    // recomputes it, and again as rd before the update: the same sums)
+    double *qTq = part + kDotBlocks, *rTr = qTq + 1, *rTr_next = qTq + 2;
     float *alpha = t.scal.as<float>(), *beta = alpha + 1;
 
     const long long *rp = t.rowptr.as<long long>();
@@ -601,9 +694,16 @@ void tomo_cgls_iterate(st_ctx *ctx, int iters) {
                            ctx->stream, rp, t.cols.as<int>(), t.vals.as<float>(), in, out, (int)N);
     };
     auto spmvt = [&](const float *in, float *out) {  // out = A^T in
-        hipLaunchKernelGGL(spmv_csr, dim3((unsigned)(((long long)n_model * 16 + 255) / 256)),
-                           dim3(256), 0, ctx->stream, cp, t.t_rows.as<int>(), t.t_vals.as<float>(),
-                           in, out, n_model);
+        const int nshort = (int)(((long long)n_model * 16 + 255) / 256);
+        hipLaunchKernelGGL(spmv_split, dim3((unsigned)(nshort + t.n_long)), dim3(256), 0,
+                           ctx->stream, cp, t.t_rows.as<int>(), t.t_vals.as<float>(), in, out,
+                           n_model, nshort, t.t_cursor.as<int>() + 1);
+    };
+    auto dot_step = [&](const float *a, long long n, double *out, const double *other,
+                        int new_is_num, float *step) {
+        hipLaunchKernelGGL(dot_partial, dim3(kDotBlocks), dim3(256), 0, ctx->stream, a, a, n, part);
+        hipLaunchKernelGGL(dot_final_ratio, dim3(1), dim3(256), 0, ctx->stream, part, kDotBlocks,
+                           out, other, new_is_num, step);
     };
     float *s = t.s.as<float>(), *q = t.q.as<float>(), *r = t.r.as<float>(), *p = t.p.as<float>(),
           *x = t.x.as<float>();
@@ -616,20 +716,17 @@ void tomo_cgls_iterate(st_ctx *ctx, int iters) {
     hipLaunchKernelGGL(copy_vec, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, p, r,
                        (long long)n_model);
     spmv(p, q);
+    if (iters > 0) dot(ctx, r, r, n_model, part, rTr);
     for (int it = 0; it < iters; it++) {
-        dot(ctx, q, q, N, part, qTq);
-        dot(ctx, r, r, n_model, part, rTr);
-        hipLaunchKernelGGL(ratio, dim3(1), dim3(64), 0, ctx->stream, rTr, qTq, alpha);
-        hipLaunchKernelGGL(axpy, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, x, p, alpha,
-                           1.0f, (long long)n_model);
-        hipLaunchKernelGGL(axpy, dim3(nblk256(N)), dim3(256), 0, ctx->stream, s, q, alpha, -1.0f, N);
-        dot(ctx, r, r, n_model, part, rd);
+        dot_step(q, N, qTq, rTr, 0, alpha);                       // alpha = rTr / qTq
+        hipLaunchKernelGGL(axpy2, dim3(nblk256(std::max<long long>(N, n_model))), dim3(256), 0,
+                           ctx->stream, x, p, (long long)n_model, 1.0f, s, q, N, -1.0f, alpha);
         spmvt(s, r);
-        dot(ctx, r, r, n_model, part, rTr);
-        hipLaunchKernelGGL(ratio, dim3(1), dim3(64), 0, ctx->stream, rTr, rd, beta);
+        dot_step(r, n_model, rTr_next, rTr, 1, beta);             // beta = rTr' / rTr
         hipLaunchKernelGGL(xpby, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream, p, r, beta,
                            (long long)n_model);
         spmv(p, q);
+        std::swap(rTr, rTr_next);
     }
     ST_HIP(hipGetLastError());
 }
