@@ -136,9 +136,17 @@ struct RayArgs {
 
 __device__ __forceinline__ int iclampd(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Workgroups are dealt to the 8 XCDs round-robin; the launch is padded to
+// 8 * per groups and group g takes the ray range (g % 8) * per + g / 8, so
+// each XCD traces a contiguous run of rays (one or two shots) and its L2
+// serves the cells those rays share near their sources.
+constexpr int kXcd = 8;
+constexpr int kRing = 16;   // runs / illumination additions a lane buffers in LDS
+
 template <class L>
-__global__ void ray_trace(RayArgs a, L lay) {
-    const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void ray_trace(RayArgs a, L lay, int per) {
+    const long long grp = (long long)(blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+    const long long ray = grp * blockDim.x + threadIdx.x;
     if (ray >= (long long)a.nshot * a.nr) return;
     const int s = (int)(ray / a.nr), r = (int)(ray - (long long)s * a.nr);
     const float *Ts = a.T + (long long)s * a.vol;
@@ -153,32 +161,75 @@ __global__ void ray_trace(RayArgs a, L lay) {
     };
     float zi = a.rcv[3 * r + 2], xi = a.rcv[3 * r], yi = a.rcv[3 * r + 1];
     int2 *my = a.runs + ray * a.max_runs;
-    int nrun = 0, cur = -1, cnt = 0;
+    // Finished runs and illumination additions wait in per-lane LDS rings
+    // and go to memory for all active lanes of the wave at once, when one
+    // lane's ring is full: a store or atomic inside the step loop would
+    // make the next step's loads wait for it to complete.
+    __shared__ int2 run_ring[kRing][64];
+    __shared__ int2 il_ring[kRing][64];
+    const int lane = threadIdx.x;   // one wave per workgroup
+    int nrun = 0, nb = 0, ni = 0;   // runs in memory, runs and additions buffered
+    auto flush = [&]() {
+        for (int u = 0; u < nb; u++) my[nrun + u] = run_ring[u][lane];
+        nrun += nb;
+        nb = 0;
+        for (int u = 0; u < ni; u++) {
+            const int2 e = il_ring[u][lane];
+            atomicAdd(&a.illum[e.x], __int_as_float(e.y));
+        }
+        ni = 0;
+    };
+    int cur = -1, cnt = 0;
     long long il_cur = -1;
     float il_acc = 0.f;
     const long long npts = (long long)a.nx * a.ny * a.nz;
     bool ok = true;
+    // padded node of a position, and the six neighbour values the central
+    // differences at that node read
+    auto node = [&](float z, float x, float y, int &i, int &j, int &k) {
+        i = iclampd((int)(z / a.dz) + 1, 1, a.nz);
+        j = iclampd((int)(x / a.dx) + 1, 1, a.nx);
+        k = iclampd((int)(y / a.dy) + 1, 1, a.ny);
+    };
+    auto load6 = [&](int i, int j, int k, float (&t)[6]) {
+        t[0] = Tp(i + 1, j, k);
+        t[1] = Tp(i - 1, j, k);
+        t[2] = Tp(i, j + 1, k);
+        t[3] = Tp(i, j - 1, k);
+        t[4] = Tp(i, j, k + 1);
+        t[5] = Tp(i, j, k - 1);
+    };
+    // Software-pipelined by one step: the next node's loads are issued as
+    // soon as the new position is known, and travel while this step's run
+    // and illumination bookkeeping runs (a node past the last step is
+    // clamped into the volume, so its loads stay in bounds).
+    int i, j, k;
+    float tv[6];
+    node(zi, xi, yi, i, j, k);
+    load6(i, j, k, tv);
     for (int step = 0;; step++) {
         if (step >= a.max_steps) { ok = false; atomicOr(a.err, 1u); break; }
-        const int i = iclampd((int)(zi / a.dz) + 1, 1, a.nz);
-        const int j = iclampd((int)(xi / a.dx) + 1, 1, a.nx);
-        const int k = iclampd((int)(yi / a.dy) + 1, 1, a.ny);
-        const float dTz = (Tp(i + 1, j, k) - Tp(i - 1, j, k)) / (2.0f * a.dz);
-        const float dTx = (Tp(i, j + 1, k) - Tp(i, j - 1, k)) / (2.0f * a.dx);
-        const float dTy = (Tp(i, j, k + 1) - Tp(i, j, k - 1)) / (2.0f * a.dy);
+        const float dTz = (tv[0] - tv[1]) / (2.0f * a.dz);
+        const float dTx = (tv[2] - tv[3]) / (2.0f * a.dx);
+        const float dTy = (tv[4] - tv[5]) / (2.0f * a.dy);
         const float norm = sqrtf(dTx * dTx + dTy * dTy + dTz * dTz);
         if (!(norm > 0.0f) || !isfinite(norm)) { ok = false; atomicOr(a.err, 1u); break; }
         zi -= rayStep * dTz / norm;
         xi -= rayStep * dTx / norm;
         yi -= rayStep * dTy / norm;
+        int i2, j2, k2;
+        float tn[6];
+        node(zi, xi, yi, i2, j2, k2);
+        load6(i2, j2, k2, tn);
+        __builtin_amdgcn_sched_barrier(0);
         const int im = iclampd((int)(zi / a.dzt), 0, a.nzt - 1);
         const int jm = iclampd((int)(xi / a.dxt), 0, a.nxt - 1);
         const int km = iclampd((int)(yi / a.dyt), 0, a.nyt - 1);
         const int cell = im + jm * a.nzt + km * a.nxt * a.nzt;
         if (cell != cur) {
             if (cnt) {
-                if (nrun >= a.max_runs) { ok = false; atomicOr(a.err, 2u); break; }
-                my[nrun++] = make_int2(cur, cnt);
+                if (nrun + nb >= a.max_runs) { ok = false; atomicOr(a.err, 2u); break; }
+                run_ring[nb++][lane] = make_int2(cur, cnt);
             }
             cur = cell;
             cnt = 0;
@@ -188,13 +239,22 @@ __global__ void ray_trace(RayArgs a, L lay) {
         // (least_squares.cpp:234); writes outside the volume are dropped
         const long long il = i + (long long)j * a.nz + (long long)k * a.nx * a.nz;
         if (il != il_cur) {
-            if (a.illum && il_cur >= 0 && il_cur < npts) atomicAdd(&a.illum[il_cur], il_acc);
+            if (a.illum && il_cur >= 0 && il_cur < npts)
+                il_ring[ni++][lane] = make_int2((int)il_cur, __float_as_int(il_acc));
             il_cur = il;
             il_acc = 0.f;
         }
         il_acc += rayStep;
-        if (cell == sId) break;
+        const bool done = cell == sId;
+        if (__ballot(nb == kRing || ni == kRing)) flush();
+        if (done) break;
+        i = i2;
+        j = j2;
+        k = k2;
+#pragma unroll
+        for (int q = 0; q < 6; q++) tv[q] = tn[q];
     }
+    flush();
     if (a.illum && il_cur >= 0 && il_cur < npts) atomicAdd(&a.illum[il_cur], il_acc);
     if (!ok) { a.nruns[ray] = 0; a.fdist[ray] = 0.f; return; }
     if (nrun >= a.max_runs) { atomicOr(a.err, 2u); a.nruns[ray] = 0; return; }
@@ -337,8 +397,11 @@ static void trace_append(st_ctx *ctx, const float *T, long long vol, L lay, cons
     a.runs = t.runs.as<int2>(); a.nruns = t.nruns.as<int>(); a.fdist = t.fdist.as<float>();
     a.illum = with_illum ? t.illum.as<float>() : nullptr;
     a.err = t.err.as<unsigned>();
-    hipLaunchKernelGGL(ray_trace<L>, dim3((unsigned)((nrays + 63) / 64)), dim3(64), 0, ctx->stream,
-                       a, lay);
+    ST_REQUIRE((long long)ctx->nx * ctx->ny * ctx->nz < (1LL << 31), ST_ERR_CAPACITY,
+               "model too large for ray tracing");
+    const int per = (int)(((nrays + 63) / 64 + kXcd - 1) / kXcd);
+    hipLaunchKernelGGL(ray_trace<L>, dim3((unsigned)(per * kXcd)), dim3(64), 0, ctx->stream, a, lay,
+                       per);
     ST_HIP(hipGetLastError());
     unsigned *herr = static_cast<unsigned *>(ctx->host_pinned) + 200;
     ST_HIP(hipMemcpyAsync(herr, t.err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
