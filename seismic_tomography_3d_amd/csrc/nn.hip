@@ -107,29 +107,37 @@ void gemm(hipStream_t st, const GemmArgs &g) {
 // One workgroup (4 waves) per BX x 4 x 4 output brick of one sample.  The
 // (BX + KS - 1) x (4 + KS - 1)^2 input halo, all channels, is staged once in
 // LDS as BF16 (row stride CINP + 4 halves to spread banks).  M-tile t holds
-// the 16 voxels x = 4 (t >> 2) + (m & 3), y = m >> 2, z = t & 3; wave w owns
-// m-tiles w and w + 4 (BX = 8) so that every B fragment feeds MT MFMAs, and
-// every A fragment NT.  K = tap * CINP + c in steps of 16
-// (v_mfma_f32_16x16x16_bf16): a lane's four consecutive k share one tap,
-// so its A fragment is one 8-byte LDS read and its B fragment one 8-byte
-// load of the packed weights Wp[tap][n][c] (n padded to 16 NT, c to CINP,
-// a zero tap absorbing the K padding of CINP = 8).  The B fragments of the
-// next tap group are loaded while the current one computes.
+// the 16 voxels x = 4 (t >> 2) + (m & 3), y = m >> 2, z = t & 3.  K = tap *
+// CINP + c in steps of 16 (v_mfma_f32_16x16x16_bf16): a lane's four
+// consecutive k share one tap, so its A fragment is one 8-byte LDS read and
+// its B fragment one 8-byte load of the fragment-ordered weights (a wave
+// reads 512 contiguous bytes; n padded to 16 NT, c to CINP, a zero tap
+// absorbing the K padding of CINP = 8).  The B fragments of a wave's next
+// tap group are loaded while the current one computes.
 // ---------------------------------------------------------------------------
-template <int KS, int CINP, int NT, int BX>
+template <int KS, int CINP, int NT, int BX, bool VEC>
 __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, int C0, int up0,
                                                   const float *__restrict__ X1, int C1,
                                                   const unsigned short *__restrict__ Wp,
                                                   const float *__restrict__ bias,
                                                   float *__restrict__ Y, int e, int cout, int relu) {
     constexpr int HF = KS / 2, HX = BX + KS - 1, H = 4 + KS - 1, NH = HX * H * H;
-    constexpr int TAPS = KS * KS * KS, CST = CINP + 4, MT = BX / 4, COUTP = 16 * NT;
+    constexpr int TAPS = KS * KS * KS, CST = CINP + 4;
     // tap groups of 16-wide k steps: CINP >= 16 -> one tap of CINP / 16
     // steps; CINP = 8 -> two taps in one step
     constexpr int GS = CINP >= 16 ? CINP / 16 : 1;
     constexpr int NG = CINP >= 16 ? TAPS : (TAPS + 1) / 2;
     static_assert(CINP == 8 || CINP % 16 == 0, "channel padding");
-    __shared__ unsigned short halo[NH * CST];
+    // KS = 3 with 4 x 4 x 4 bricks (the wide-input layers): the tap groups
+    // are dealt to the 4 waves (g = w, w + 4, ...), each wave computes all 4
+    // m-tiles over its taps (every weight fragment is read by one wave of
+    // the workgroup), and the partial sums meet in LDS in a fixed order.
+    // Otherwise wave w owns m-tiles w and w + 4 over all taps (measured
+    // faster for the narrow 8 x 4 x 4 bricks).
+    constexpr bool KSPLIT = KS > 1 && BX == 4;
+    constexpr int MT = KSPLIT ? BX : BX / 4, GSTEP = KSPLIT ? 4 : 1;
+    constexpr int NHALO = NH * CST, NPART = KSPLIT ? 1024 * MT * NT : 0;  // halves
+    __shared__ __align__(16) unsigned short halo[NHALO > NPART ? NHALO : NPART];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nbx = (e + BX - 1) / BX, nb4 = (e + 3) >> 2;
     int bid = blockIdx.x;
@@ -143,39 +151,55 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
     const int ox = BX * bx - HF, oy = 4 * by - HF, oz = 4 * bz - HF;
     const int cin = C0 + C1;
 
-    // stage the halo, 4 channels per item
+    // stage the halo, 4 channels per item, SU items per thread in flight:
+    // every load of a group is issued before the first one is converted.
+    // VEC: both inputs have a multiple of 4 channels (one 16-byte load per
+    // item); otherwise the channels are read one by one.
     constexpr int C4 = CINP / 4;
-    for (int i = tid; i < NH * C4; i += 256) {
-        const int hv = i / C4, c = (i - hv * C4) * 4;
-        const int hx = hv % HX, hy = (hv / HX) % H, hz = hv / (HX * H);
-        const int xx = ox + hx, yy = oy + hy, zz = oz + hz;
-        float f[4] = {0.f, 0.f, 0.f, 0.f};
-        if (c < cin && xx >= 0 && yy >= 0 && zz >= 0 && xx < e && yy < e && zz < e) {
-            const float *p;
-            int cc, nav;
-            if (c < C0) {
-                p = up0 ? X0 + ((long long)s * Vh + (xx >> 1) + (long long)eh * ((yy >> 1) + (long long)eh * (zz >> 1))) * C0
-                        : X0 + ((long long)s * V + xx + (long long)e * (yy + (long long)e * zz)) * C0;
-                cc = c;
-                nav = C0 - c;
-            } else {
-                p = X1 + ((long long)s * V + xx + (long long)e * (yy + (long long)e * zz)) * C1;
-                cc = c - C0;
-                nav = C1 - cc;
-            }
-            if (nav >= 4 && ((C0 | C1) & 3) == 0) {
-                const float4 v = *reinterpret_cast<const float4 *>(p + cc);
-                f[0] = v.x; f[1] = v.y; f[2] = v.z; f[3] = v.w;
-            } else {
+    constexpr int NI = NH * C4, SU = 8;
+    for (int base = 0; base < NI; base += 256 * SU) {
+        float f[SU][4];
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (k < nav) f[k] = p[cc + k];
+        for (int u = 0; u < SU; u++) {
+            const int i = base + u * 256 + tid;
+            f[u][0] = f[u][1] = f[u][2] = f[u][3] = 0.f;
+            if (i >= NI) continue;
+            const int hv = i / C4, c = (i - hv * C4) * 4;
+            const int hx = hv % HX, hy = (hv / HX) % H, hz = hv / (HX * H);
+            const int xx = ox + hx, yy = oy + hy, zz = oz + hz;
+            if (c < cin && xx >= 0 && yy >= 0 && zz >= 0 && xx < e && yy < e && zz < e) {
+                const float *p;
+                int cc, nav;
+                if (c < C0) {
+                    p = up0 ? X0 + ((long long)s * Vh + (xx >> 1) + (long long)eh * ((yy >> 1) + (long long)eh * (zz >> 1))) * C0
+                            : X0 + ((long long)s * V + xx + (long long)e * (yy + (long long)e * zz)) * C0;
+                    cc = c;
+                    nav = C0 - c;
+                } else {
+                    p = X1 + ((long long)s * V + xx + (long long)e * (yy + (long long)e * zz)) * C1;
+                    cc = c - C0;
+                    nav = C1 - cc;
+                }
+                if constexpr (VEC) {
+                    const float4 v = *reinterpret_cast<const float4 *>(p + cc);
+                    f[u][0] = v.x; f[u][1] = v.y; f[u][2] = v.z; f[u][3] = v.w;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (k < nav) f[u][k] = p[cc + k];
+                }
             }
         }
-        uint2 u;
-        u.x = (unsigned)f2bf(f[0]) | ((unsigned)f2bf(f[1]) << 16);
-        u.y = (unsigned)f2bf(f[2]) | ((unsigned)f2bf(f[3]) << 16);
-        *reinterpret_cast<uint2 *>(&halo[hv * CST + c]) = u;
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int i = base + u * 256 + tid;
+            if (i >= NI) continue;
+            const int hv = i / C4, c = (i - hv * C4) * 4;
+            uint2 w2;
+            w2.x = (unsigned)f2bf(f[u][0]) | ((unsigned)f2bf(f[u][1]) << 16);
+            w2.y = (unsigned)f2bf(f[u][2]) | ((unsigned)f2bf(f[u][3]) << 16);
+            *reinterpret_cast<uint2 *>(&halo[hv * CST + c]) = w2;
+        }
     }
     __syncthreads();
 
@@ -184,7 +208,7 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
     int hb[MT];
 #pragma unroll
     for (int t = 0; t < MT; t++) {
-        const int mt = w + 4 * t;
+        const int mt = KSPLIT ? t : w + 4 * t;
         const int vx = 4 * (mt >> 2) + (m & 3), vy = m >> 2, vz = mt & 3;
         hb[t] = vx + HX * (vy + H * vz);
     }
@@ -200,17 +224,17 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
 
     s4v bcur[GS][NT], bnxt[GS][NT];
     auto load_b = [&](int g, s4v (&b)[GS][NT]) {
-        const int tap = tap_of(g);
 #pragma unroll
         for (int k = 0; k < GS; k++)
 #pragma unroll
             for (int j = 0; j < NT; j++)
                 b[k][j] = *reinterpret_cast<const s4v *>(
-                    Wp + ((long long)(tap * COUTP + j * 16 + m)) * CINP + k * 16 + cq);
+                    Wp + ((((long long)g * GS + k) * NT + j) * 64 + lane) * 4);
     };
-    load_b(0, bcur);
-    for (int g = 0; g < NG; g++) {
-        if (g + 1 < NG) load_b(g + 1, bnxt);
+    const int g0 = KSPLIT ? w : 0;
+    if (g0 < NG) load_b(g0, bcur);
+    for (int g = g0; g < NG; g += GSTEP) {
+        if (g + GSTEP < NG) load_b(g + GSTEP, bnxt);
         int tap = tap_of(g);
         if (tap >= TAPS) tap = 0;  // padding tap: its weights are zero
         const int tx = tap % KS, ty = (tap / KS) % KS, tz = tap / (KS * KS);
@@ -233,9 +257,41 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
             for (int j = 0; j < NT; j++) bcur[k][j] = bnxt[k][j];
     }
 
+    if constexpr (KSPLIT) {
+        // partial sums: waves 2, 3 -> waves 0, 1, then wave 1 -> wave 0
+        float *part = reinterpret_cast<float *>(halo);
+        constexpr int PT = MT * NT * 4;
+        auto put = [&](int slot) {
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        part[(slot * PT + (t * NT + j) * 4 + r) * 64 + lane] = acc[t][j][r];
+        };
+        auto add = [&](int slot) {
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        acc[t][j][r] += part[(slot * PT + (t * NT + j) * 4 + r) * 64 + lane];
+        };
+        __syncthreads();  // every wave is done with the halo
+        if (w >= 2) put(w - 2);
+        __syncthreads();
+        if (w < 2) add(w);
+        __syncthreads();
+        if (w == 1) put(0);
+        __syncthreads();
+        if (w != 0) return;
+        add(0);
+    }
 #pragma unroll
     for (int t = 0; t < MT; t++) {
-        const int mt = w + 4 * t;
+        const int mt = KSPLIT ? t : w + 4 * t;
 #pragma unroll
         for (int j = 0; j < NT; j++) {
             const int n = j * 16 + m;
@@ -255,17 +311,28 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
     }
 }
 
-// Wp[tap][n][c] (taps + 1 taps, n < 16*ceil(cout/16), c < cinp) from
-// W16[tap*cin + c][cout]; zero where tap == taps, n >= cout or c >= cin.
+// Wp in MFMA fragment order: fragment (g, k, j) is 64 lanes x 4 BF16,
+// lane = m + 16 q holding W[tap][c .. c+3][n = 16 j + m] with, for cinp >= 16,
+// tap = g and c = 16 k + 4 q (k < cinp / 16), and for cinp = 8 (two taps per
+// group) tap = 2 g + (q >> 1) and c = 4 (q & 1); zero where tap >= taps,
+// n >= cout or c >= cin.  A wave's B fragment is one contiguous 512-byte
+// read.  W16 is [tap*cin + c][cout].
 __global__ void pack_conv(const unsigned short *__restrict__ W16, unsigned short *__restrict__ Wp,
                           int taps, int cin, int cout, int cinp, int coutp) {
-    const long long tot = (long long)(taps + 1) * coutp * cinp;
+    const int gs = cinp >= 16 ? cinp / 16 : 1, ng = cinp >= 16 ? taps : (taps + 1) / 2;
+    const int nt = coutp / 16;
+    const long long tot = (long long)ng * gs * nt * 256;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
          i += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % cinp);
-        const long long r = i / cinp;
-        const int n = (int)(r % coutp);
-        const int tap = (int)(r / coutp);
+        const int e4 = (int)(i & 3), lane = (int)((i >> 2) & 63);
+        long long f = i >> 8;
+        const int j = (int)(f % nt);
+        f /= nt;
+        const int k = (int)(f % gs);
+        const int g = (int)(f / gs);
+        const int m = lane & 15, q = lane >> 4, n = 16 * j + m;
+        const int tap = cinp >= 16 ? g : 2 * g + (q >> 1);
+        const int c = cinp >= 16 ? 16 * k + 4 * q + e4 : 4 * (q & 1) + e4;
         unsigned short v = 0;
         if (tap < taps && n < cout && c < cin) v = W16[((long long)tap * cin + c) * cout + n];
         Wp[i] = v;
@@ -571,11 +638,16 @@ void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const floa
     if (nwg == 0) return;
     const dim3 g((unsigned)nwg), b(256);
     const int u = up0 ? 1 : 0, r = relu ? 1 : 0;
+    const bool vec = ((C0 | C1) & 3) == 0;  // 16-byte aligned channel groups
 #define ST_CONV(KS, CP, NT, BXV)                                                                  \
     if (ksz == KS && cinp == CP && coutp == 16 * NT) {                                            \
         ST_REQUIRE(bx == BXV, ST_ERR_ARG, "conv brick");                                          \
-        hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV>), g, b, 0, st, X0, C0, u, X1, C1, Wp,     \
-                           bias, Y, e, cout, r);                                                  \
+        if (vec)                                                                                  \
+            hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV, true>), g, b, 0, st, X0, C0, u, X1,   \
+                               C1, Wp, bias, Y, e, cout, r);                                      \
+        else                                                                                      \
+            hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV, false>), g, b, 0, st, X0, C0, u, X1,  \
+                               C1, Wp, bias, Y, e, cout, r);                                      \
         ST_HIP(hipGetLastError());                                                                \
         return;                                                                                   \
     }

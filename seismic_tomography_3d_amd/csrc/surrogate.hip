@@ -331,13 +331,18 @@ __device__ __forceinline__ float tri_fine(const float *V, int nx, int ny, int nz
                                           float fz) {
     const int i0 = min((int)fx, nx - 2), j0 = min((int)fy, ny - 2), k0 = min((int)fz, nz - 2);
     const float u = fx - i0, v = fy - j0, w = fz - k0;
-    auto at = [&](int i, int j, int k) {
-        return V[k + (long long)i * nz + (long long)j * nx * nz];
+    // the two z neighbours of a corner are adjacent (z fastest, k0 + 1 < nz):
+    // one 8-byte load each (dword alignment suffices for global loads)
+    struct __attribute__((aligned(4))) Pair { float lo, hi; };
+    auto at2 = [&](int i, int j) {
+        return *reinterpret_cast<const Pair *>(V + k0 + (long long)i * nz + (long long)j * nx * nz);
     };
-    const float c00 = at(i0, j0, k0) * (1 - u) + at(i0 + 1, j0, k0) * u;
-    const float c10 = at(i0, j0 + 1, k0) * (1 - u) + at(i0 + 1, j0 + 1, k0) * u;
-    const float c01 = at(i0, j0, k0 + 1) * (1 - u) + at(i0 + 1, j0, k0 + 1) * u;
-    const float c11 = at(i0, j0 + 1, k0 + 1) * (1 - u) + at(i0 + 1, j0 + 1, k0 + 1) * u;
+    const Pair p00 = at2(i0, j0), p10 = at2(i0 + 1, j0), p01 = at2(i0, j0 + 1),
+               p11 = at2(i0 + 1, j0 + 1);
+    const float c00 = p00.lo * (1 - u) + p10.lo * u;
+    const float c10 = p01.lo * (1 - u) + p11.lo * u;
+    const float c01 = p00.hi * (1 - u) + p10.hi * u;
+    const float c11 = p01.hi * (1 - u) + p11.hi * u;
     return ((c00 * (1 - v) + c10 * v) * (1 - w) + (c01 * (1 - v) + c11 * v) * w);
 }
 
@@ -355,18 +360,23 @@ __device__ __forceinline__ void node_pos(const CubeGeo &g, int v, float &px, flo
     px = fx * g.dx; py = fy * g.dy; pz = fz * g.dz;
 }
 
-// X[s, v, 0..4] from the fine slowness (z fastest) and the sources
+// X[s, v, 0..4] from the fine slowness (z fastest) and the sources.
+// Threads walk the nodes z fastest (the node index v is x fastest), so the
+// lanes of a wave sample neighbouring z columns of the fine volume, which is
+// z fastest in memory: their gathers share cache lines.
 __global__ void prep_inputs(const float *__restrict__ slow, const float *__restrict__ src, int ns,
                             CubeGeo g, float *__restrict__ X) {
     const long long V = cube(g.nc);
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ns * V;
          i += (long long)gridDim.x * blockDim.x) {
-        const int s = (int)(i / V), v = (int)(i - s * V);
+        const int s = (int)(i / V), w = (int)(i - s * V);
+        const int wz = w % g.nc, wx = (w / g.nc) % g.nc, wy = w / (g.nc * g.nc);
+        const int v = wx + g.nc * (wy + g.nc * wz);
         float px, py, pz, fx, fy, fz;
         node_pos(g, v, px, py, pz, fx, fy, fz);
         const float ex = px - src[3 * s], ey = py - src[3 * s + 1], ez = pz - src[3 * s + 2];
         const float D = sqrtf(ex * ex + ey * ey + ez * ez);
-        float *o = X + i * kIn;
+        float *o = X + ((long long)s * V + v) * kIn;
         const float sn = tri_fine(slow, g.nx, g.ny, g.nz, fx, fy, fz);
         // straight-ray mean slowness (first arrivals never exceed it)
         const float gx = src[3 * s] / g.dx, gy = src[3 * s + 1] / g.dy, gz = src[3 * s + 2] / g.dz;
