@@ -3,6 +3,8 @@
 // average pooling, nearest upsampling, channel concat, bias/ReLU, MSE loss
 // and Adam.  Activations are channels-last [sample][z][y][x][c], voxel
 // v = x + n*(y + n*z) on an n^3 cube.
+#include <type_traits>
+
 #include "nn.hpp"
 
 namespace st {
@@ -115,12 +117,16 @@ void gemm(hipStream_t st, const GemmArgs &g) {
 // absorbing the K padding of CINP = 8).  The B fragments of a wave's next
 // tap group are loaded while the current one computes.
 // ---------------------------------------------------------------------------
-template <int KS, int CINP, int NT, int BX, bool VEC>
-__global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, int C0, int up0,
-                                                  const float *__restrict__ X1, int C1,
+template <int KS, int CINP, int NT, int BX, bool VEC, bool IBF, bool OBF>
+__global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, int C0, int up0,
+                                                  const void *__restrict__ X1v, int C1,
                                                   const unsigned short *__restrict__ Wp,
                                                   const float *__restrict__ bias,
-                                                  float *__restrict__ Y, int e, int cout, int relu) {
+                                                  void *__restrict__ Yv, int e, int cout, int relu) {
+    // activations in FP32, or in BF16 (IBF: inputs, OBF: output)
+    using TI = typename std::conditional<IBF, unsigned short, float>::type;
+    const TI *__restrict__ X0 = static_cast<const TI *>(X0v);
+    const TI *__restrict__ X1 = static_cast<const TI *>(X1v);
     constexpr int HF = KS / 2, HX = BX + KS - 1, H = 4 + KS - 1, NH = HX * H * H;
     constexpr int TAPS = KS * KS * KS, CST = CINP + 4;
     // tap groups of 16-wide k steps: CINP >= 16 -> one tap of CINP / 16
@@ -158,17 +164,19 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
     constexpr int C4 = CINP / 4;
     constexpr int NI = NH * C4, SU = 8;
     for (int base = 0; base < NI; base += 256 * SU) {
-        float f[SU][4];
+        float f[SU][4];      // FP32 inputs
+        uint2 raw[SU];       // BF16 inputs, 4 channels as loaded
 #pragma unroll
         for (int u = 0; u < SU; u++) {
             const int i = base + u * 256 + tid;
             f[u][0] = f[u][1] = f[u][2] = f[u][3] = 0.f;
+            raw[u] = make_uint2(0u, 0u);
             if (i >= NI) continue;
             const int hv = i / C4, c = (i - hv * C4) * 4;
             const int hx = hv % HX, hy = (hv / HX) % H, hz = hv / (HX * H);
             const int xx = ox + hx, yy = oy + hy, zz = oz + hz;
             if (c < cin && xx >= 0 && yy >= 0 && zz >= 0 && xx < e && yy < e && zz < e) {
-                const float *p;
+                const TI *p;
                 int cc, nav;
                 if (c < C0) {
                     p = up0 ? X0 + ((long long)s * Vh + (xx >> 1) + (long long)eh * ((yy >> 1) + (long long)eh * (zz >> 1))) * C0
@@ -180,7 +188,18 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
                     cc = c - C0;
                     nav = C1 - cc;
                 }
-                if constexpr (VEC) {
+                if constexpr (IBF) {
+                    if constexpr (VEC) {
+                        raw[u] = *reinterpret_cast<const uint2 *>(p + cc);
+                    } else {
+                        unsigned short hb4[4] = {0, 0, 0, 0};
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            if (k < nav) hb4[k] = p[cc + k];
+                        raw[u].x = (unsigned)hb4[0] | ((unsigned)hb4[1] << 16);
+                        raw[u].y = (unsigned)hb4[2] | ((unsigned)hb4[3] << 16);
+                    }
+                } else if constexpr (VEC) {
                     const float4 v = *reinterpret_cast<const float4 *>(p + cc);
                     f[u][0] = v.x; f[u][1] = v.y; f[u][2] = v.z; f[u][3] = v.w;
                 } else {
@@ -196,8 +215,12 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
             if (i >= NI) continue;
             const int hv = i / C4, c = (i - hv * C4) * 4;
             uint2 w2;
-            w2.x = (unsigned)f2bf(f[u][0]) | ((unsigned)f2bf(f[u][1]) << 16);
-            w2.y = (unsigned)f2bf(f[u][2]) | ((unsigned)f2bf(f[u][3]) << 16);
+            if constexpr (IBF) {
+                w2 = raw[u];
+            } else {
+                w2.x = (unsigned)f2bf(f[u][0]) | ((unsigned)f2bf(f[u][1]) << 16);
+                w2.y = (unsigned)f2bf(f[u][2]) | ((unsigned)f2bf(f[u][3]) << 16);
+            }
             *reinterpret_cast<uint2 *>(&halo[hv * CST + c]) = w2;
         }
     }
@@ -305,7 +328,11 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const float *__restrict__ X0, 
                 if (x >= e || y >= e || z >= e) continue;
                 float v = acc[t][j][r] + bn;
                 if (relu) v = fmaxf(v, 0.f);
-                Y[((long long)s * V + x + (long long)e * (y + (long long)e * z)) * cout + n] = v;
+                const long long o = ((long long)s * V + x + (long long)e * (y + (long long)e * z)) * cout + n;
+                if constexpr (OBF)
+                    static_cast<unsigned short *>(Yv)[o] = f2bf(v);
+                else
+                    static_cast<float *>(Yv)[o] = v;
             }
         }
     }
@@ -409,6 +436,26 @@ __global__ void avgpool2(const float *__restrict__ X, float *__restrict__ Y, int
             acc += X[(s * V + xx + (long long)n * (yy + (long long)n * zz)) * C + c];
         }
         Y[i] = 0.125f * acc;
+    }
+}
+
+// the same on BF16 activations (FP32 sum, rounded once)
+__global__ void avgpool2_bf16(const unsigned short *__restrict__ X, unsigned short *__restrict__ Y,
+                              int ns, int n, int C) {
+    const int m = n / 2;
+    const long long V2 = (long long)m * m * m, V = (long long)n * n * n;
+    GRID_STRIDE(i, (long long)ns * V2 * C) {
+        const int c = (int)(i % C);
+        const long long sv = i / C;
+        const long long s = sv / V2;
+        const int v = (int)(sv - s * V2);
+        const int x = v % m, y = (v / m) % m, z = v / (m * m);
+        float acc = 0.f;
+        for (int d = 0; d < 8; d++) {
+            const int xx = 2 * x + (d & 1), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d >> 2);
+            acc += bf2f(X[(s * V + xx + (long long)n * (yy + (long long)n * zz)) * C + c]);
+        }
+        Y[i] = f2bf(0.125f * acc);
     }
 }
 
@@ -565,6 +612,10 @@ void pool(hipStream_t st, const float *X, float *Y, int ns, int n, int C) {
     hipLaunchKernelGGL(avgpool2, dim3(blocks_for((long long)ns * n * n * n / 8 * C)), dim3(256), 0,
                        st, X, Y, ns, n, C);
 }
+void pool_bf16(hipStream_t st, const unsigned short *X, unsigned short *Y, int ns, int n, int C) {
+    hipLaunchKernelGGL(avgpool2_bf16, dim3(blocks_for((long long)ns * n * n * n / 8 * C)), dim3(256),
+                       0, st, X, Y, ns, n, C);
+}
 void pool_bwd(hipStream_t st, const float *dY, float *dX, int ns, int n, int C) {
     hipLaunchKernelGGL(avgpool2_bwd, dim3(blocks_for((long long)ns * n * n * n * C)), dim3(256), 0,
                        st, dY, dX, ns, n, C);
@@ -622,9 +673,9 @@ void pack_conv_weights(hipStream_t st, const unsigned short *W16, unsigned short
     ST_HIP(hipGetLastError());
 }
 
-void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const float *X1, int C1,
-                   const unsigned short *Wp, const float *bias, float *Y, int ns, int e, int cout,
-                   int ksz, bool relu) {
+void conv_forward_any(hipStream_t st, const void *X0, int C0, bool up0, const void *X1, int C1,
+                      bool in_bf16, const unsigned short *Wp, const float *bias, void *Y,
+                      bool out_bf16, int ns, int e, int cout, int ksz, bool relu) {
     const int cin = C0 + C1;
     ST_REQUIRE(ksz == 1 || ksz == 3, ST_ERR_ARG, "conv kernel size must be 1 or 3");
     ST_REQUIRE(C0 >= 1 && C1 >= 0 && (C1 == 0 || X1) && cout >= 1 && cout <= 64 && e >= 1 &&
@@ -638,16 +689,23 @@ void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const floa
     if (nwg == 0) return;
     const dim3 g((unsigned)nwg), b(256);
     const int u = up0 ? 1 : 0, r = relu ? 1 : 0;
-    const bool vec = ((C0 | C1) & 3) == 0;  // 16-byte aligned channel groups
+    const bool vec = ((C0 | C1) & 3) == 0;  // 16-byte (8-byte for BF16) channel groups
+    const int mode = (vec ? 4 : 0) | (in_bf16 ? 2 : 0) | (out_bf16 ? 1 : 0);
+#define ST_CONV_MODE(KS, CP, NT, BXV, M)                                                          \
+    case M:                                                                                       \
+        hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV, ((M) & 4) != 0, ((M) & 2) != 0,           \
+                                       ((M) & 1) != 0>),                                          \
+                           g, b, 0, st, X0, C0, u, X1, C1, Wp, bias, Y, e, cout, r);              \
+        break;
 #define ST_CONV(KS, CP, NT, BXV)                                                                  \
     if (ksz == KS && cinp == CP && coutp == 16 * NT) {                                            \
         ST_REQUIRE(bx == BXV, ST_ERR_ARG, "conv brick");                                          \
-        if (vec)                                                                                  \
-            hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV, true>), g, b, 0, st, X0, C0, u, X1,   \
-                               C1, Wp, bias, Y, e, cout, r);                                      \
-        else                                                                                      \
-            hipLaunchKernelGGL((conv3d_fwd<KS, CP, NT, BXV, false>), g, b, 0, st, X0, C0, u, X1,  \
-                               C1, Wp, bias, Y, e, cout, r);                                      \
+        switch (mode) {                                                                           \
+            ST_CONV_MODE(KS, CP, NT, BXV, 0) ST_CONV_MODE(KS, CP, NT, BXV, 1)                     \
+            ST_CONV_MODE(KS, CP, NT, BXV, 2) ST_CONV_MODE(KS, CP, NT, BXV, 3)                     \
+            ST_CONV_MODE(KS, CP, NT, BXV, 4) ST_CONV_MODE(KS, CP, NT, BXV, 5)                     \
+            ST_CONV_MODE(KS, CP, NT, BXV, 6) ST_CONV_MODE(KS, CP, NT, BXV, 7)                     \
+        }                                                                                         \
         ST_HIP(hipGetLastError());                                                                \
         return;                                                                                   \
     }
@@ -662,7 +720,14 @@ void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const floa
     ST_CONV(3, 128, 4, 4)
     ST_CONV(1, 16, 1, 8)
 #undef ST_CONV
+#undef ST_CONV_MODE
     ST_REQUIRE(false, ST_ERR_ARG, "no convolution kernel for this channel configuration");
+}
+
+void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const float *X1, int C1,
+                   const unsigned short *Wp, const float *bias, float *Y, int ns, int e, int cout,
+                   int ksz, bool relu) {
+    conv_forward_any(st, X0, C0, up0, X1, C1, false, Wp, bias, Y, false, ns, e, cout, ksz, relu);
 }
 
 void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, const float *bias,

@@ -17,6 +17,8 @@ __host__ __device__ inline unsigned short f2bf(float f) {  // round to nearest e
     return (unsigned short)(u >> 16);
 }
 
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
 struct GemmArgs {
     const unsigned short *A;
     const unsigned short *B;
@@ -33,6 +35,7 @@ void gemm(hipStream_t st, const GemmArgs &g);
 void im2col(hipStream_t st, const float *X, unsigned short *col, int ns, int n, int C, int ksz);
 void col2im(hipStream_t st, const float *dcol, float *dX, int ns, int n, int C, int ksz);
 void pool(hipStream_t st, const float *X, float *Y, int ns, int n, int C);
+void pool_bf16(hipStream_t st, const unsigned short *X, unsigned short *Y, int ns, int n, int C);
 void pool_bwd(hipStream_t st, const float *dY, float *dX, int ns, int n, int C);
 void upsample(hipStream_t st, const float *X, float *Y, int ns, int m, int C, int Ctot, int off);
 void upsample_bwd(hipStream_t st, const float *dY, float *dX, int ns, int m, int C, int Ctot, int off);
@@ -60,6 +63,10 @@ void conv_forward(hipStream_t st, const float *X, const unsigned short *Wp, cons
 void conv_forward2(hipStream_t st, const float *X0, int C0, bool up0, const float *X1, int C1,
                    const unsigned short *Wp, const float *bias, float *Y, int ns, int e, int cout,
                    int ksz, bool relu);
+// the same with FP32 or BF16 (in_bf16, out_bf16) activations
+void conv_forward_any(hipStream_t st, const void *X0, int C0, bool up0, const void *X1, int C1,
+                      bool in_bf16, const unsigned short *Wp, const float *bias, void *Y,
+                      bool out_bf16, int ns, int e, int cout, int ksz, bool relu);
 
 }  // namespace nn
 }  // namespace st

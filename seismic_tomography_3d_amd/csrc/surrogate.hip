@@ -189,13 +189,15 @@ static void conv_fwd(st_ctx *ctx, Net &n, int li, const float *X, float *Y, int 
                      ns, n.nc / c.res_div, c.cin, c.cout, c.ksz, c.relu);
 }
 
-// decoder convolution on [upsample(Xup) (Cup channels) | Xskip (Cskip)]
-static void conv_cat(st_ctx *ctx, Net &n, int li, const float *Xup, int Cup, const float *Xskip,
-                     int Cskip, float *Y, int ns) {
+// one convolution of the inference pass on [X0 (nearest-upsampled from half
+// resolution when up0) | X1], activations in FP32 or BF16
+static void conv_any(st_ctx *ctx, Net &n, int li, const void *X0, int C0, bool up0, const void *X1,
+                     int C1, bool in_bf16, void *Y, bool out_bf16, int ns) {
     const Conv &c = n.L[li];
-    ST_REQUIRE(Cup + Cskip == c.cin, ST_ERR_ARG, "concat channels do not match the layer");
-    nn::conv_forward2(ctx->stream, Xup, Cup, true, Xskip, Cskip, n.wp.as<unsigned short>() + c.op,
-                      n.w.as<float>() + c.ob, Y, ns, n.nc / c.res_div, c.cout, c.ksz, c.relu);
+    ST_REQUIRE(C0 + C1 == c.cin, ST_ERR_ARG, "input channels do not match the layer");
+    nn::conv_forward_any(ctx->stream, X0, C0, up0, X1, C1, in_bf16, n.wp.as<unsigned short>() + c.op,
+                         n.w.as<float>() + c.ob, Y, out_bf16, ns, n.nc / c.res_div, c.cout, c.ksz,
+                         c.relu);
 }
 
 // re-packs the BF16 weights for conv3d_fwd after any change of n.w16
@@ -256,7 +258,38 @@ static void add(st_ctx *ctx, float *y, const float *x, long long n) {
 // deeper activation and the skip activation directly (no concat buffers);
 // the backward pass needs the materialised concats, so training and
 // gradient evaluation run unfused.
-static void net_forward(st_ctx *ctx, Net &n, int ns, bool fused = false) {
+// Inference pass: BF16 activations between layers (the convolutions stage
+// their inputs as BF16 MFMA operands either way; only the 2x2x2 means are
+// taken of BF16 rather than FP32 values), FP32 network input and output,
+// and the decoder's upsampling and skip concatenation done while staging.
+static void net_infer(st_ctx *ctx, Net &n, int ns) {
+    hipStream_t st = ctx->stream;
+    const int e1 = n.nc, e2 = e1 / 2, e4 = e1 / 4;
+    net_pack(ctx, n);
+    auto h = [](DevBuf &b) { return b.as<unsigned short>(); };
+    conv_any(ctx, n, 0, n.in0.ptr, kIn, false, nullptr, 0, false, h(n.a1), true, ns);
+    conv_any(ctx, n, 1, h(n.a1), 16, false, nullptr, 0, true, h(n.a2), true, ns);
+    nn::pool_bf16(st, h(n.a2), h(n.p1), ns, e1, 16);
+    conv_any(ctx, n, 2, h(n.p1), 16, false, nullptr, 0, true, h(n.a3), true, ns);
+    conv_any(ctx, n, 3, h(n.a3), 32, false, nullptr, 0, true, h(n.a4), true, ns);
+    nn::pool_bf16(st, h(n.a4), h(n.p2), ns, e2, 32);
+    conv_any(ctx, n, 4, h(n.p2), 32, false, nullptr, 0, true, h(n.a5), true, ns);
+    nn::pool_bf16(st, h(n.a5), h(n.p3), ns, e4, 64);
+    conv_any(ctx, n, 5, h(n.p3), 64, false, nullptr, 0, true, h(n.a7), true, ns);
+    conv_any(ctx, n, 6, h(n.a7), 64, true, h(n.a5), 64, true, h(n.a8), true, ns);
+    conv_any(ctx, n, 7, h(n.a8), 64, true, h(n.a4), 32, true, h(n.a9), true, ns);
+    conv_any(ctx, n, 8, h(n.a9), 32, true, h(n.a2), 16, true, h(n.a10), true, ns);
+    conv_any(ctx, n, 9, h(n.a10), 16, false, nullptr, 0, true, n.out.ptr, false, ns);
+    ST_HIP(hipGetLastError());
+}
+
+// inference (infer: net_infer) or the FP32 pass whose activations the
+// backward pass reads
+static void net_forward(st_ctx *ctx, Net &n, int ns, bool infer = false) {
+    if (infer) {
+        net_infer(ctx, n, ns);
+        return;
+    }
     hipStream_t st = ctx->stream;
     const int e1 = n.nc, e2 = e1 / 2, e4 = e1 / 4, e8 = e1 / 8;
     net_pack(ctx, n);
@@ -269,21 +302,15 @@ static void net_forward(st_ctx *ctx, Net &n, int ns, bool fused = false) {
     conv_fwd(ctx, n, 4, n.p2.as<float>(), n.a5.as<float>(), ns);
     nn::pool(st, n.a5.as<float>(), n.p3.as<float>(), ns, e4, 64);
     conv_fwd(ctx, n, 5, n.p3.as<float>(), n.a7.as<float>(), ns);
-    if (fused) {
-        conv_cat(ctx, n, 6, n.a7.as<float>(), 64, n.a5.as<float>(), 64, n.a8.as<float>(), ns);
-        conv_cat(ctx, n, 7, n.a8.as<float>(), 64, n.a4.as<float>(), 32, n.a9.as<float>(), ns);
-        conv_cat(ctx, n, 8, n.a9.as<float>(), 32, n.a2.as<float>(), 16, n.a10.as<float>(), ns);
-    } else {
-        nn::upsample(st, n.a7.as<float>(), n.cat3.as<float>(), ns, e8, 64, 128, 0);
-        nn::chancopy(st, n.a5.as<float>(), n.cat3.as<float>(), cube(e4) * ns, 64, 128, 64, false);
-        conv_fwd(ctx, n, 6, n.cat3.as<float>(), n.a8.as<float>(), ns);
-        nn::upsample(st, n.a8.as<float>(), n.cat2.as<float>(), ns, e4, 64, 96, 0);
-        nn::chancopy(st, n.a4.as<float>(), n.cat2.as<float>(), cube(e2) * ns, 32, 96, 64, false);
-        conv_fwd(ctx, n, 7, n.cat2.as<float>(), n.a9.as<float>(), ns);
-        nn::upsample(st, n.a9.as<float>(), n.cat1.as<float>(), ns, e2, 32, 48, 0);
-        nn::chancopy(st, n.a2.as<float>(), n.cat1.as<float>(), cube(e1) * ns, 16, 48, 32, false);
-        conv_fwd(ctx, n, 8, n.cat1.as<float>(), n.a10.as<float>(), ns);
-    }
+    nn::upsample(st, n.a7.as<float>(), n.cat3.as<float>(), ns, e8, 64, 128, 0);
+    nn::chancopy(st, n.a5.as<float>(), n.cat3.as<float>(), cube(e4) * ns, 64, 128, 64, false);
+    conv_fwd(ctx, n, 6, n.cat3.as<float>(), n.a8.as<float>(), ns);
+    nn::upsample(st, n.a8.as<float>(), n.cat2.as<float>(), ns, e4, 64, 96, 0);
+    nn::chancopy(st, n.a4.as<float>(), n.cat2.as<float>(), cube(e2) * ns, 32, 96, 64, false);
+    conv_fwd(ctx, n, 7, n.cat2.as<float>(), n.a9.as<float>(), ns);
+    nn::upsample(st, n.a9.as<float>(), n.cat1.as<float>(), ns, e2, 32, 48, 0);
+    nn::chancopy(st, n.a2.as<float>(), n.cat1.as<float>(), cube(e1) * ns, 16, 48, 32, false);
+    conv_fwd(ctx, n, 8, n.cat1.as<float>(), n.a10.as<float>(), ns);
     conv_fwd(ctx, n, 9, n.a10.as<float>(), n.out.as<float>(), ns);
     ST_HIP(hipGetLastError());
 }
@@ -503,7 +530,7 @@ bool surrogate_warm_fill(st_ctx *ctx, const float *dsrc, int nshot, int B, float
     const long long V = cube(n.nc);
     const CubeGeo g = cube_geo(ctx, n.nc);
     n.Q.reserve(sizeof(float) * V * nshot);
-    const int chunk = 128;
+    const int chunk = 256;
     net_alloc_batch(n, std::min(chunk, nshot), false);
     for (int s0 = 0; s0 < nshot; s0 += chunk) {
         const int ns = std::min(chunk, nshot - s0);

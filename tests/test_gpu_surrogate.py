@@ -54,6 +54,21 @@ def test_network_forward_matches_fp64(ctx, nc, ns):
     assert err < 5e-2
 
 
+@pytest.mark.parametrize("nc,ns", [(8, 2), (16, 3), (32, 2)])
+def test_inference_pass_matches_fp32_pass(ctx, nc, ns):
+    """The inference pass (BF16 activations between layers, upsampling and
+    skip concatenation while staging) against the FP32 pass the training
+    uses, on the same weights and inputs: the two differ only by the BF16
+    rounding of activations before the 2x2x2 means."""
+    ctx.surrogate_init(nc, seed=4)
+    X, T, M = _inputs(nc, ns, 2)
+    Yi, _, _ = ctx.surrogate_eval(X)
+    Yf, _, _ = ctx.surrogate_eval(X, T, M)
+    err = np.linalg.norm(Yi - Yf) / np.linalg.norm(Yf)
+    print(f"inference vs FP32 pass rel diff: {err:.2e}")
+    assert err < 5e-3
+
+
 def test_network_gradient_vs_finite_differences(ctx):
     nc, ns = 8, 2
     n = ctx.surrogate_init(nc, seed=4)
