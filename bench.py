@@ -198,12 +198,17 @@ def tomography_leg(with_cpu):
     ctx.model_update(None, smooth=True, samples=5, stdv=2.0)
     t3 = time.perf_counter()
     rp, cols, vals = ctx.csr()
+    nrays = int(src.shape[0] * rcv.shape[0])
+    # ray steps: every step adds rayStep (0.2 h) to its cell's entry (the
+    # source cell's entry is the final distance instead, under one step)
+    ray_steps = float(np.sum(vals[:rp[nrays]], dtype=np.float64) / (0.2 * h))
     out = {"workload": "16 shots x 2,500 receivers, 201x201x101 at 12.5 m, 41x41x21 inversion "
                        "grid, order 2, lambda 1e-2, 20 CGLS iterations, Gaussian 5/2.0",
+           "ray_steps_est": ray_steps, "G_nnz": int(rp[nrays]),
            "gpu_ms": {"forward": 1e3 * (t1 - t0), "forward_fim_device": fst["solve_ms"],
                       "optimize": 1e3 * (t2 - t1), "model_update": 1e3 * (t3 - t2),
                       "iteration": 1e3 * (t3 - t0)},
-           "rays": int(src.shape[0] * rcv.shape[0]), "nnz_with_tikhonov": int(rp[-1])}
+           "rays": nrays}
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as orc
