@@ -58,6 +58,10 @@ typedef struct {
                            than delta; the judge re-solves the others cold),
                            0 (default) keep T cold and use the prediction only
                            to order block visits (exact by construction)       */
+    int sweep;          /* 8^3 blocks: 0 (default) = each visit opens with one
+                           Gauss-Seidel pass over diagonal planes in the octant
+                           facing away from the source, then z sweeps; 1 = z
+                           sweeps only                                          */
 } st_fim_opts;
 
 typedef struct {

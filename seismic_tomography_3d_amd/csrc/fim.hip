@@ -59,6 +59,7 @@ struct FimArgs {
     unsigned *kmin;            // [3][nshot] per-shot smallest key of each list (float bits)
     int nshot;
     float band;
+    const float *src;          // the batch's sources, 3 per shot (x, y, z in m)
 };
 
 constexpr unsigned kInfBits = 0x7F800000u;
@@ -338,11 +339,21 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round4(FimArgs A) {
 // 8^3 blocks: lane = tx + 8 ty owns the z column tz = 0..7 in registers.
 // x/y neighbours come through the LDS tile, z neighbours from registers
 // (Gauss-Seidel along z, alternating direction each sweep).
+// HYP: each visit first makes one full Gauss-Seidel pass in the octant
+// pointing away from the shot's source -- cells taken by diagonal planes
+// x' + y' + z' = d (coordinates flipped into the octant), 22 planes of at
+// most 64 cells, one lane per column; a cell's upwind neighbours lie on
+// plane d - 1, updated one step before -- so a front crossing the block
+// in that octant is carried through it in one pass; the z sweeps then
+// confirm or finish the fixed point.
 // ---------------------------------------------------------------------------
+template <bool HYP>
 __global__ __launch_bounds__(64 * kWaves) void fim_round8(FimArgs A) {
     __shared__ float tiles[kWaves][1000];  // (8+2)^3, index x + 10 y + 100 z
+    __shared__ float hss[HYP ? kWaves : 1][512];  // h * slowness, tx + 8 ty + 64 z
     const int lane = threadIdx.x & 63;
     float *tile = tiles[threadIdx.x >> 6];
+    float *hsl = hss[HYP ? (threadIdx.x >> 6) : 0];
     const unsigned gw = blockIdx.x * kWaves + (threadIdx.x >> 6), G = gridDim.x * kWaves;
     const int p = A.round & 1;
     unsigned incl = 0, count = 0;
@@ -403,9 +414,54 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round8(FimArgs A) {
             }
             tile[ti] = hv;
         }
+        int it = 0;
+        if constexpr (HYP) {
+            tile[tcol] = vz[0];
+            tile[tcol + 900] = vz[9];
+#pragma unroll
+            for (int k = 0; k < 8; k++) hsl[lane + 64 * k] = hs[k];
+            wave_sync();
+            // octant: along each axis the front enters through the face
+            // holding the earlier arrivals (diving waves reach shallow blocks
+            // from below); on a tie, away from the source
+            auto wmin = [](float v) {
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+                return v;
+            };
+            const int fu = lane & 7, fw = lane >> 3;
+            const float mxm = wmin(tile[10 * (fu + 1) + 100 * (fw + 1)]);
+            const float mxp = wmin(tile[9 + 10 * (fu + 1) + 100 * (fw + 1)]);
+            const float mym = wmin(tile[(fu + 1) + 100 * (fw + 1)]);
+            const float myp = wmin(tile[(fu + 1) + 90 + 100 * (fw + 1)]);
+            const float mzm = wmin(vz[0]), mzp = wmin(vz[9]);
+            const float *sp = A.src + 3 * shot;
+            const bool fx = mxm != mxp ? mxm < mxp : (bx * 8 + 3.5f) * A.h >= sp[0];
+            const bool fy = mym != myp ? mym < myp : (by * 8 + 3.5f) * A.h >= sp[1];
+            const bool fz = mzm != mzp ? mzm < mzp : (bz * 8 + 3.5f) * A.h >= sp[2];
+            const int xl = fx ? tx : 7 - tx, yl = fy ? ty : 7 - ty;
+            const int zmax = A.n4z - bz * 8;  // cells at or past it are outside the model
+            for (int d = 0; d < 22; d++) {
+                const int zl = d - xl - yl;
+                if (zl >= 0 && zl < 8) {
+                    const int k = fz ? zl : 7 - zl;
+                    const int i = tcol + 100 * (k + 1);
+                    const float own = tile[i];
+                    const float a = fminf(tile[i - 1], tile[i + 1]);
+                    const float bb = fminf(tile[i - 10], tile[i + 10]);
+                    const float c = fminf(tile[i - 100], tile[i + 100]);
+                    const float hk = hsl[lane + 64 * k];
+                    const float t = godunov(a, bb, c, hk, hk * hk);
+                    if (vxy && k < zmax) tile[i] = fminf(own, t);
+                }
+                wave_sync();
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) vz[k + 1] = tile[tcol + 100 * (k + 1)];
+            it = 1;
+        }
         wave_sync();
 
-        int it = 0;
         bool more = true;
         // dirty: z planes whose inputs changed since their last update.  A
         // plane reads its own plane (x/y neighbours, Jacobi) and planes k - 1
@@ -811,6 +867,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     A.kmin = ctx->kmin.as<unsigned>();
     A.nshot = nshot;
     A.band = o.band;
+    A.src = dsrc;
 
     // o.grid counts waves; a launch has o.grid / kWaves workgroups
     int grid = o.grid;
@@ -834,6 +891,15 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         return t;
     };
     const long long max_rounds = 4LL * (g.nbx + g.nby + g.nbz) * 64 + 4096;
+    const bool hyp = o.sweep != 1;
+    auto launch_round = [&](const FimArgs &a) {
+        if (B == 8 && hyp)
+            hipLaunchKernelGGL(fim_round8<true>, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, a);
+        else if (B == 8)
+            hipLaunchKernelGGL(fim_round8<false>, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, a);
+    };
 
     ST_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     long long r = 0;
@@ -853,6 +919,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         mix(nshot); mix(B); mix(grid); mix(A.nblk); mix(A.max_inner);
         mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
         mix((long long)__builtin_bit_cast(unsigned, A.h));
+        mix((long long)(uintptr_t)A.src); mix(hyp ? 1 : 0);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) ST_HIP(hipGraphExecDestroy(ctx->graph_exec));
             if (ctx->graph) ST_HIP(hipGraphDestroy(ctx->graph));
@@ -861,10 +928,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
             ST_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < 6; k++) {
                 A.round = k;
-                if (B == 8)
-                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
-                else
-                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
+                launch_round(A);
             }
             ST_HIP(hipStreamEndCapture(ctx->stream, &ctx->graph));
             ST_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
@@ -879,10 +943,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         } else {
             for (int k = 0; k < Kg; k++, r++) {
                 A.round = (int)(r % 6);
-                if (B == 8)
-                    hipLaunchKernelGGL(fim_round8, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
-                else
-                    hipLaunchKernelGGL(fim_round4, dim3(grid), dim3(64 * kWaves), 0, ctx->stream, A);
+                launch_round(A);
             }
         }
         ST_HIP(hipGetLastError());

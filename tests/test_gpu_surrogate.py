@@ -139,12 +139,18 @@ def test_training_and_warm_start_gain(ctx, tmp_path):
     assert sw["fallbacks"] == 0
     # the gain: with a visit cap below what a block needs to converge, cold
     # solves revisit blocks; visiting them in predicted-arrival order removes
-    # most revisits (at this coarse grid an uncapped 8^3 visit already
-    # converges its block, the full-size case is measured by bench.py)
-    capped = dict(batch=32, block=8, max_inner=8)
+    # most revisits.  Shown with z sweeps only (sweep = 1), where 8 sweeps are
+    # below what a block needs; the default visit opens with a diagonal pass
+    # that converges most blocks within the cap at this coarse grid (an
+    # uncapped visit converges its block either way; the full-size gain of
+    # the default solver is measured by bench.py)
+    capped = dict(batch=32, block=8, max_inner=8, sweep=1)
     _, Tc8, sc8 = ctx.fim_solve(src, rcv, return_T=True, **capped)
     _, Tw8, sw8 = ctx.fim_solve(src, rcv, return_T=True, warm=1, **capped)
     print(f"max_inner 8: cold iters {sc8['inner_iters']} visits {sc8['visits']}; ordered iters "
           f"{sw8['inner_iters']} visits {sw8['visits']}")
-    assert np.abs(Tw8 - Tc8).max() < 1e-5
+    # both stop within eps = 1e-6 s per update of the fixed point; with the
+    # visit cap the two visit orders stop at different points of that band,
+    # which adds up along a path by about eps per block crossed (<= 14 here)
+    assert np.abs(Tw8 - Tc8).max() < 3e-5
     assert sw8["inner_iters"] < 0.7 * sc8["inner_iters"]
