@@ -33,7 +33,7 @@ def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=256, log=Non
     if candidates is None:
         candidates = []
         for B in ([block] if block else [4, 8]):
-            for wpc in ([grid // CUS] if grid else ([16] if B == 4 else [16, 32])):
+            for wpc in ([grid // CUS] if grid else ([16] if B == 4 else [16, 32, 48])):
                 for mi in ([max_inner] if max_inner else ([16] if B == 4 else [16, 24])):
                     candidates.append(dict(block=B, grid=wpc * CUS, max_inner=mi))
     best, best_ms = None, float("inf")
@@ -72,16 +72,17 @@ def best_warm_params(ctx, src, rcv, cold, sample=256, log=None):
     # one block crossing at 1500 m/s as the unit of the band
     unit = cold["block"] * h / 1500.0
     best, best_ms = None, float("inf")
-    for mi in sorted({cold["max_inner"], 24}):
-        for f in (1.0, 1.5, 2.0):
-            o = dict(cold, batch=n, warm=1, band=f * unit, max_inner=mi)
-            ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
-            _, _, st = ctx.fim_solve(sub, rcv, **o)
-            ms = st["total_ms"]
-            if log:
-                log(f"tune warm max_inner {mi} band {f * unit:.3f}s: {ms:.2f} ms "
-                    f"(solve {st['solve_ms']:.2f}) for {n} shots, {st['visits']} visits, "
-                    f"{st['fallbacks']} fallbacks")
-            if ms < best_ms:
-                best, best_ms = dict(band=f * unit, max_inner=mi), ms
+    grids = sorted({cold["grid"], 48 * CUS}) if cold["block"] == 8 else [cold["grid"]]
+    caps = sorted({cold["max_inner"], 24})
+    for g, mi, f in ((g, mi, f) for g in grids for mi in caps for f in (1.0, 1.5, 2.0)):
+        o = dict(cold, batch=n, warm=1, band=f * unit, max_inner=mi, grid=g)
+        ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
+        _, _, st = ctx.fim_solve(sub, rcv, **o)
+        ms = st["total_ms"]
+        if log:
+            log(f"tune warm grid {g} max_inner {mi} band {f * unit:.3f}s: {ms:.2f} ms "
+                f"(solve {st['solve_ms']:.2f}) for {n} shots, {st['visits']} visits, "
+                f"{st['fallbacks']} fallbacks")
+        if ms < best_ms:
+            best, best_ms = dict(band=f * unit, max_inner=mi, grid=g), ms
     return best
