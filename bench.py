@@ -285,6 +285,62 @@ def tomography_leg(with_cpu):
 
 
 
+def large_inversion(iters=10, log=None):
+    """The 401x401x201 (h = 10 m, 32 M cells) inversion: 1,024 surface shots
+    on a 32x32 grid, 4,096 receivers on a 64x64 grid at 100 m, picks from
+    the true model (5 seeded layers + 3 anomalies, seed 2) by the FIM;
+    start = lateral average; 81x81x41 inversion grid (50 m); per iteration
+    a forward pass in chunks of 64 shots (FIM + picks + ray tracing into
+    the CSR matrix), order-2 Tikhonov CGLS of 30 iterations and the model
+    update with the 5-sample Gaussian as one captured graph.  Reports data
+    RMS misfit and model RMS error per iteration."""
+    from seismic_tomography_3d_amd import Context, synth
+    nx, ny, nz, h = 401, 401, 201, 10.0
+    t0 = time.perf_counter()
+    vel_true = synth.layered_model(nx, ny, nz, h, n_layers=5, n_anom=3, seed=2)
+    vel0 = synth.lateral_average(vel_true)
+    L = (nx - 1) * h
+    sx, sy, sz = synth.regular_geometry((0.05 * L, 0.05 * L), (0.05 * L, 0.95 * L),
+                                        (0.95 * L, 0.05 * L), 32, 32, 0.0)
+    src = np.stack([sx, sy, sz], 1).astype(np.float32)
+    rcv = np.stack(synth.surface_grid(64, 0.02 * L, 0.98 * L, 100.0), 1).astype(np.float32)
+    if log:
+        log(f"large inversion: models built in {time.perf_counter() - t0:.1f} s")
+    ctx = Context(0)
+    ctx.set_model(1.0 / vel_true, h)
+    a = time.perf_counter()
+    dobs, _, _ = ctx.fim_solve(src, rcv, batch=64)
+    t_obs = time.perf_counter() - a
+    ctx.set_model(1.0 / vel0, h)
+    nt = ctx.tomo_set_grid(50.0)
+    rows = []
+    for it in range(iters + 1):
+        a = time.perf_counter()
+        dcal, _, fst = ctx.forward(src, rcv, batch=64)
+        t_fwd = time.perf_counter() - a
+        misfit = ctx.residual_norm(dobs, dcal)
+        vel = 1.0 / ctx.get_model()
+        merr = float(np.sqrt(np.mean((vel.astype(np.float64) - vel_true) ** 2)))
+        row = {"iteration": it, "data_rms_s": misfit / np.sqrt(dobs.size), "model_rms_mps": merr,
+               "forward_s": t_fwd, "fim_device_s": fst["solve_ms"] / 1e3}
+        if it < iters:
+            a = time.perf_counter()
+            ctx.optimize_update(dobs, dcal, 2, 1e-2, 30, smooth=True, samples=5, stdv=2.0,
+                                graph=True, fetch=False)
+            row["solve_update_s"] = time.perf_counter() - a
+        if it == 0:
+            row["G_rows"], row["G_nnz"] = ctx.csr_size()
+        rows.append(row)
+        if log:
+            log(f"large inversion {row}")
+    ctx.close()
+    return {"workload": "401x401x201 at 10 m, 1,024 shots x 4,096 receivers (4.19 M rays), "
+                        "81x81x41 grid, order 2, lambda 1e-2, 30 CGLS iterations, Gaussian 5/2.0, "
+                        "chunks of 64 shots",
+            "observed_picks_s": t_obs, "iterations": rows,
+            "tomo_dims": list(nt)}
+
+
 def synthesis_leg(with_cpu, traffic_file, nt=3001):
     """Data synthesis at the reference's wave_main.cpp / auto_picking.cpp
     sizes: 701x501x101 at 10 m with 50 absorbing cells per side (801x601x201
@@ -383,6 +439,8 @@ def main():
                     help="run only the data-synthesis leg (acoustic modelling + picking)")
     ap.add_argument("--tomography-only", action="store_true",
                     help="only run the tomography-iteration leg")
+    ap.add_argument("--large-inversion", type=int, default=0, metavar="ITERS",
+                    help="only run ITERS iterations of the 401x401x201 inversion")
     ap.add_argument("--synthesis-nt", type=int, default=3001,
                     help="time steps of the data-synthesis leg (3001 as the reference)")
     ap.add_argument("--steps", type=int, default=3)
@@ -416,6 +474,9 @@ def main():
 
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.large_inversion:
+        print(json.dumps({"large_inversion": large_inversion(a.large_inversion, log)}), flush=True)
         return
     if a.tomography_only:
         print(json.dumps({"tomography_iteration": tomography_leg(not a.no_cpu)}), flush=True)

@@ -241,6 +241,12 @@ class Context:
                                                 rcv.shape[0], _ptr(il)))
         return il
 
+    def csr_size(self):
+        """(rows, non-zeros) of the device sensitivity matrix."""
+        rows, nnz = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self.lib.st_csr_size(self.h, ctypes.byref(rows), ctypes.byref(nnz)))
+        return rows.value, nnz.value
+
     def csr(self):
         rows, nnz = ctypes.c_longlong(), ctypes.c_longlong()
         self._check(self.lib.st_csr_size(self.h, ctypes.byref(rows), ctypes.byref(nnz)))

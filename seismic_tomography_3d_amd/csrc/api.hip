@@ -226,6 +226,7 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
     if (o.eps < 0.f) o.eps = kFimEps;
     if (o.delta <= 0.f) o.delta = 0.02f;
     ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
+    ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
     const bool warm = (o.warm == 1) || T_pred;
     if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
     if (o.band < 0.f || !warm) o.band = 1e30f;

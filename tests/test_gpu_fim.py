@@ -19,15 +19,17 @@ def _case(nx=41, ny=37, nz=21, h=25.0, seed=1):
     return vel, h
 
 
-@pytest.mark.parametrize("block", [4, 8])
-def test_fim_small_vs_fp64(ctx, block):
+@pytest.mark.parametrize("block,sweep", [(4, 0), (8, 0), (8, 1)])
+def test_fim_small_vs_fp64(ctx, block, sweep):
+    """8^3 visits with the diagonal-plane pass (sweep 0, the default) and
+    with z sweeps only (sweep 1) against the FP64 fixed point."""
     vel, h = _case()
     ctx.set_model(1.0 / vel, h)
     srcs = np.array([[500.0, 437.3, 110.0], [0.0, 0.0, 0.0], [1000.0, 900.0, 500.0],
                      [12.5, 880.0, 260.0]], np.float32)
     rcv = np.stack(synth.surface_grid(6, 25.0, 875.0, 0.0), axis=1)
-    picks, T, st = ctx.fim_solve(srcs, rcv, return_T=True, block=block, batch=4)
-    print(f"block={block} stats={st}")
+    picks, T, st = ctx.fim_solve(srcs, rcv, return_T=True, block=block, batch=4, sweep=sweep)
+    print(f"block={block} sweep={sweep} stats={st}")
     for i, s in enumerate(srcs):
         T64 = orc.eikonal_fp64(vel, h, s)
         err = np.abs(T[i].astype(np.float64) - T64).max()
@@ -113,6 +115,8 @@ def test_rejects_source_outside(ctx):
         ctx.fim_solve(np.array([[-1.0, 0.0, 0.0]], np.float32), None)
     with pytest.raises(StError):
         ctx.fim_solve(np.array([[0.0, 0.0, 1e5]], np.float32), None)
+    with pytest.raises(StError):
+        ctx.fim_solve(np.array([[10.0, 10.0, 10.0]], np.float32), None, sweep=2)
 
 
 def test_graph_replay_matches_eager(ctx):
