@@ -285,7 +285,7 @@ def tomography_leg(with_cpu):
 
 
 
-def large_inversion(iters=10, log=None):
+def large_inversion(iters=10, log=None, warm=False):
     """The 401x401x201 (h = 10 m, 32 M cells) inversion: 1,024 surface shots
     on a 32x32 grid, 4,096 receivers on a 64x64 grid at 100 m, picks from
     the true model (5 seeded layers + 3 anomalies, seed 2) by the FIM;
@@ -293,7 +293,9 @@ def large_inversion(iters=10, log=None):
     a forward pass in chunks of 64 shots (FIM + picks + ray tracing into
     the CSR matrix), order-2 Tikhonov CGLS of 30 iterations and the model
     update with the 5-sample Gaussian as one captured graph.  Reports data
-    RMS misfit and model RMS error per iteration."""
+    RMS misfit and model RMS error per iteration.  warm: order the FIM
+    visits by the warm-start network (the network of the 5 x 5 x 2.5 km
+    workload, applied to this 4 x 4 x 2 km model; slower here than cold)."""
     from seismic_tomography_3d_amd import Context, synth
     nx, ny, nz, h = 401, 401, 201, 10.0
     t0 = time.perf_counter()
@@ -307,16 +309,19 @@ def large_inversion(iters=10, log=None):
     if log:
         log(f"large inversion: models built in {time.perf_counter() - t0:.1f} s")
     ctx = Context(0)
+    from seismic_tomography_3d_amd import tune
+    warm = 1 if warm and tune.load_surrogate(ctx, None, log=log) else 0
+    fo = dict(batch=64, warm=warm)
     ctx.set_model(1.0 / vel_true, h)
     a = time.perf_counter()
-    dobs, _, _ = ctx.fim_solve(src, rcv, batch=64)
+    dobs, _, _ = ctx.fim_solve(src, rcv, **fo)
     t_obs = time.perf_counter() - a
     ctx.set_model(1.0 / vel0, h)
     nt = ctx.tomo_set_grid(50.0)
     rows = []
     for it in range(iters + 1):
         a = time.perf_counter()
-        dcal, _, fst = ctx.forward(src, rcv, batch=64)
+        dcal, _, fst = ctx.forward(src, rcv, **fo)
         t_fwd = time.perf_counter() - a
         misfit = ctx.residual_norm(dobs, dcal)
         vel = 1.0 / ctx.get_model()
@@ -337,7 +342,7 @@ def large_inversion(iters=10, log=None):
     return {"workload": "401x401x201 at 10 m, 1,024 shots x 4,096 receivers (4.19 M rays), "
                         "81x81x41 grid, order 2, lambda 1e-2, 30 CGLS iterations, Gaussian 5/2.0, "
                         "chunks of 64 shots",
-            "observed_picks_s": t_obs, "iterations": rows,
+            "warm_start": bool(warm), "observed_picks_s": t_obs, "iterations": rows,
             "tomo_dims": list(nt)}
 
 
@@ -441,6 +446,8 @@ def main():
                     help="only run the tomography-iteration leg")
     ap.add_argument("--large-inversion", type=int, default=0, metavar="ITERS",
                     help="only run ITERS iterations of the 401x401x201 inversion")
+    ap.add_argument("--large-warm", action="store_true",
+                    help="order the large inversion's FIM visits by the warm-start network")
     ap.add_argument("--synthesis-nt", type=int, default=3001,
                     help="time steps of the data-synthesis leg (3001 as the reference)")
     ap.add_argument("--steps", type=int, default=3)
@@ -476,7 +483,8 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.large_inversion:
-        print(json.dumps({"large_inversion": large_inversion(a.large_inversion, log)}), flush=True)
+        print(json.dumps({"large_inversion": large_inversion(a.large_inversion, log, a.large_warm)}),
+              flush=True)
         return
     if a.tomography_only:
         print(json.dumps({"tomography_iteration": tomography_leg(not a.no_cpu)}), flush=True)

@@ -152,7 +152,9 @@ int st_tomo_dims(st_ctx *ctx, int *nx_tomo, int *ny_tomo, int *nz_tomo);
  * shot the eikonal solve, the receiver picks (dcal, nshot*nrcv) and the
  * gradient ray tracing (:192-280) into the device-resident sensitivity
  * matrix G (CSR, row = shot*nrcv + receiver, columns ascending).  illum
- * (nullable, nx*ny*nz) receives the ray illumination summed over shots. */
+ * (nullable, nx*ny*nz) receives the ray illumination summed over shots.
+ * opts->warm = 1 orders the block visits by the warm-start network's
+ * prediction (warm_values must stay 0: the result is the cold fixed point). */
 int st_forward(st_ctx *ctx, const float *src_xyz, int nshot,
                const float *rcv_xyz, int nrcv, float *dcal, float *illum,
                const st_fim_opts *opts, st_fim_stats *stats);

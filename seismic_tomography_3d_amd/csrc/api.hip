@@ -583,7 +583,15 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
         ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
         if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 24 : 16;
         if (o.eps < 0.f) o.eps = kFimEps;
-        ST_REQUIRE(o.warm == 0, ST_ERR_ARG, "forward runs cold starts");
+        ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
+        // a forward pass must reach the cold fixed point: a warm start here
+        // only orders the block visits by the network's prediction
+        ST_REQUIRE(o.warm == 0 || o.warm_values == 0, ST_ERR_ARG,
+                   "forward warm starts order visits only (warm_values = 0)");
+        ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
+        const bool warm = o.warm == 1;
+        if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
+        if (o.band < 0.f || !warm) o.band = 1e30f;
         if (stats) std::memset(stats, 0, sizeof(*stats));
         check_points(src_xyz, nshot, ctx, "source");
         check_points(rcv_xyz, nrcv, ctx, "receiver");
@@ -599,15 +607,25 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
         ctx->src.reserve(sizeof(float) * 3 * nshot);
         ctx->rcv.reserve(sizeof(float) * 3 * nrcv);
         ctx->picks.reserve(sizeof(float) * (long long)nshot * nrcv);
+        if (warm) ctx->keys.reserve(sizeof(float) * g.nblk * batch);
         ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
                               hipMemcpyHostToDevice, ctx->stream));
         ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
                               hipMemcpyHostToDevice, ctx->stream));
+        const float r_cold = o.cold_radius > 0.f ? o.cold_radius
+                                                 : (warm ? surrogate_cold_radius(ctx) : 0.f);
         tomo_reset(ctx, true);
         for (int b0 = 0; b0 < nshot; b0 += batch) {
             const int nb = std::min(batch, nshot - b0);
             const float *dsrc = ctx->src.as<float>() + 3LL * b0;
-            fim_batch(ctx, dsrc, nb, o, stats, false, nullptr);
+            const float *keys = nullptr;
+            if (warm) {
+                const bool ok = surrogate_warm_fill(ctx, dsrc, nb, B, o.delta, r_cold, false,
+                                                    ctx->keys.as<float>());
+                ST_REQUIRE(ok, ST_ERR_NO_SURROGATE, "warm start requested without a network");
+                keys = ctx->keys.as<float>();
+            }
+            fim_batch(ctx, dsrc, nb, o, stats, warm, keys);
             launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, ctx->rcv.as<float>(), nrcv,
                              ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx, ctx->ny,
                              ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
