@@ -443,6 +443,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             const bool fz = mzm != mzp ? mzm < mzp : (bz * 8 + 3.5f) * A.h >= sp[2];
             const int xl = fx ? tx : 7 - tx, yl = fy ? ty : 7 - ty;
             const int zmax = A.n4z - bz * 8;  // cells at or past it are outside the model
+#pragma unroll
             for (int d = 0; d < 22; d++) {
                 const int zl = d - xl - yl;
                 if (zl >= 0 && zl < 8) {
