@@ -78,9 +78,9 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 // sequences.  The 3D discriminant is 3(hs)^2 - [(a-b)^2 + (b-c)^2 + (c-a)^2]
 // in sorted differences d = mid - lo, e1 = hi - mid, e3 = hi - lo = d + e1:
 // the bracket equals 2 (d e3 + e1^2), with no cancellation at large T (the
-// reference's expanded 2(a(b-a)+b(c-b)+c(a-c)) + 3(hs)^2 has it).  Square
-// roots take max(., 0): a negative argument belongs to a candidate that is
-// not selected, so no NaN is ever formed.  Unreached neighbours are the
+// reference's expanded 2(a(b-a)+b(c-b)+c(a-c)) + 3(hs)^2 has it).  No square
+// root ever sees a negative argument, so no NaN is formed: the 3D one takes
+// max(., 0), the 2D one the absolute value (exact whenever it is selected).  Unreached neighbours are the
 // finite sentinel kInf; a cell whose neighbours are all kInf gets a value
 // above kInf, which min(T, .) discards.
 __device__ __forceinline__ float godunov(float a, float b, float c, float hs, float hs2) {
@@ -90,7 +90,9 @@ __device__ __forceinline__ float godunov(float a, float b, float c, float hs, fl
     const float t1 = lo + hs;
     const float d = mid - lo;
     const float sm = mid + lo;
-    const float t2 = 0.5f * (sm + fsqrt(fmaxf(2.0f * hs2 - d * d, 0.0f)));
+    // |.| (a free source modifier of v_sqrt): whenever t2 is selected, t1 =
+    // lo + hs > mid, so d < hs (up to rounding) and the argument exceeds hs^2
+    const float t2 = 0.5f * (sm + fsqrt(fabsf(2.0f * hs2 - d * d)));
     const float e1 = hi - mid, e3 = hi - lo;
     const float u = d * e3 + e1 * e1;
     const float t3 = ((hi + sm) + fsqrt(fmaxf(3.0f * hs2 - 2.0f * u, 0.0f))) * (1.0f / 3.0f);
