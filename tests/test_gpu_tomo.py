@@ -71,6 +71,7 @@ def test_forward_builds_G_and_picks(ctx):
     assert np.abs(dcal - picks).max() < 1e-6
     rp, c, v = ctx.csr()
     assert len(rp) == len(src) * len(rcv) + 1
+    assert ctx.csr_size() == (len(rp) - 1, int(rp[-1]))
     same = 0
     total = 0
     oill = np.zeros_like(ill)
