@@ -873,6 +873,81 @@ __global__ void gaussian3d(const float *__restrict__ in, float *__restrict__ out
     out[index] = accum;
 }
 
+// The same filter for a compile-time window S, tiled: a 256-thread
+// workgroup owns a 32 (z) x 8 (x) column of outputs and marches along y
+// through kGaussYC planes, keeping the S input planes the current output
+// plane reads (with their S/2 halos, zero outside the volume) in an LDS
+// ring, so each input value comes from memory about once per column
+// instead of S^3 times.  Every interior cell adds its S^3 products in the
+// reference's (yk, xk, zk) order, multiply then add, as gaussian3d does.
+constexpr int kGaussTZ = 32, kGaussTX = 8, kGaussYC = 16;
+template <int S>
+__global__ __launch_bounds__(256) void gaussian3d_tiled(const float *__restrict__ in,
+                                                        float *__restrict__ out,
+                                                        const float *__restrict__ kern, int nx,
+                                                        int ny, int nz) {
+    constexpr int H = S / 2, PZ = kGaussTZ + S - 1, PX = kGaussTX + S - 1, PL = PZ * PX;
+    __shared__ float ring[S][PL];
+    const int tz = threadIdx.x % kGaussTZ, tx = threadIdx.x / kGaussTZ;
+    const int z0 = blockIdx.x * kGaussTZ, x0 = blockIdx.y * kGaussTX, y0 = blockIdx.z * kGaussYC;
+    const int y1 = min(y0 + kGaussYC, ny);
+    const int i = z0 + tz, j = x0 + tx;
+    const long long pstride = (long long)nx * nz;
+    // input plane yp with its halo into ring slot `slot`
+    auto load = [&](int yp, int slot) {
+        for (int e = threadIdx.x; e < PL; e += 256) {
+            const int zz = z0 - H + e % PZ, xx = x0 - H + e / PZ;
+            float v = 0.0f;
+            if (yp >= 0 && yp < ny && zz >= 0 && zz < nz && xx >= 0 && xx < nx)
+                v = in[zz + (long long)xx * nz + (long long)yp * pstride];
+            ring[slot][e] = v;
+        }
+    };
+    // plane p sits in slot (p - y0 + H) mod S
+    for (int q = 0; q < S - 1; q++) load(y0 - H + q, q);
+    for (int y = y0; y < y1; y++) {
+        load(y + H, (y - y0 + S - 1) % S);
+        __syncthreads();
+        if (i < nz && j < nx) {
+            const long long o = i + (long long)j * nz + (long long)y * pstride;
+            const int base = (y - y0) % S;  // slot of plane y - H
+            if (y < H || y >= ny - H || j < H || j >= nx - H || i < H || i >= nz - H) {
+                out[o] = ring[(base + H) % S][(tx + H) * PZ + tz + H];
+            } else {
+                float accum = 0.0f;
+#pragma unroll
+                for (int yk = 0; yk < S; yk++) {
+                    const float *pl = ring[(base + yk) % S] + tx * PZ + tz;
+#pragma unroll
+                    for (int xk = 0; xk < S; xk++)
+#pragma unroll
+                        for (int zk = 0; zk < S; zk++)
+                            accum += pl[xk * PZ + zk] * kern[zk + xk * S + yk * S * S];
+                }
+                out[o] = accum;
+            }
+        }
+        __syncthreads();  // the next plane overwrites slot `base`
+    }
+}
+
+static void launch_gaussian(hipStream_t st, const float *in, float *out, const float *kern, int nx,
+                            int ny, int nz, int samples) {
+    const long long n = (long long)nx * ny * nz;
+    const dim3 g((nz + kGaussTZ - 1) / kGaussTZ, (nx + kGaussTX - 1) / kGaussTX,
+                 (ny + kGaussYC - 1) / kGaussYC);
+    ST_REQUIRE(g.y <= 65535u && g.z <= 65535u, ST_ERR_ARG, "model too large for the filter grid");
+    switch (samples) {
+    case 3: hipLaunchKernelGGL(gaussian3d_tiled<3>, g, dim3(256), 0, st, in, out, kern, nx, ny, nz); break;
+    case 5: hipLaunchKernelGGL(gaussian3d_tiled<5>, g, dim3(256), 0, st, in, out, kern, nx, ny, nz); break;
+    case 7: hipLaunchKernelGGL(gaussian3d_tiled<7>, g, dim3(256), 0, st, in, out, kern, nx, ny, nz); break;
+    case 9: hipLaunchKernelGGL(gaussian3d_tiled<9>, g, dim3(256), 0, st, in, out, kern, nx, ny, nz); break;
+    default:
+        hipLaunchKernelGGL(gaussian3d, dim3(nblk256(n)), dim3(256), 0, st, in, out, kern, nx, ny, nz,
+                           samples);
+    }
+}
+
 // slowness += dm inside the margins (:553-565), velocity = 1 / slowness
 __global__ void apply_update(float *__restrict__ slow, const float *__restrict__ dm,
                              float *__restrict__ vel, int nx, int ny, int nz) {
@@ -923,8 +998,7 @@ static float *update_kernels(st_ctx *ctx, TomoState &t, const float *dxdm, int s
                        t.nxt, t.nyt, t.nzt, t.dxt, t.dyt, t.dzt);
     float *dm = t.dm.as<float>(), *tmp = t.upd.as<float>();
     if (smooth) {
-        hipLaunchKernelGGL(gaussian3d, dim3(nblk256(n)), dim3(256), 0, ctx->stream, dm, tmp, kern,
-                           ctx->nx, ctx->ny, ctx->nz, samples);
+        launch_gaussian(ctx->stream, dm, tmp, kern, ctx->nx, ctx->ny, ctx->nz, samples);
         std::swap(dm, tmp);
     }
     hipLaunchKernelGGL(apply_update, dim3(nblk256(n)), dim3(256), 0, ctx->stream,
@@ -1043,8 +1117,8 @@ void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, 
         t.kern.reserve(sizeof(float) * kern.size());
         ST_HIP(hipMemcpyAsync(t.kern.ptr, kern.data(), sizeof(float) * kern.size(),
                               hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(gaussian3d, dim3(nblk256(n)), dim3(256), 0, ctx->stream, dm,
-                           t.upd.as<float>(), t.kern.as<float>(), ctx->nx, ctx->ny, ctx->nz, samples);
+        launch_gaussian(ctx->stream, dm, t.upd.as<float>(), t.kern.as<float>(), ctx->nx, ctx->ny,
+                        ctx->nz, samples);
         ST_HIP(hipStreamSynchronize(ctx->stream));  // kern is reused; keep host vector alive
         std::swap(t.dm, t.upd);
         dm = t.dm.as<float>();

@@ -140,9 +140,12 @@ def test_optimize_vs_fp64_recurrence(ctx):
     assert r == pytest.approx(np.sqrt(np.sum((dobs.astype(np.float64) - dcal) ** 2)), rel=1e-12)
 
 
-@pytest.mark.parametrize("smooth", [False, True])
-def test_model_update_bit_exact(ctx, smooth):
-    nx, ny, nz, h = 60, 56, 30, 10.0
+# windows 3-9 run the LDS-tiled filter, 11 the one-cell-per-thread one; the
+# extents are ragged against the 32 (z) x 8 (x) x 16 (y) tiles
+@pytest.mark.parametrize("smooth,samples", [(False, 5), (True, 3), (True, 5), (True, 7), (True, 9),
+                                            (True, 11)])
+def test_model_update_bit_exact(ctx, smooth, samples):
+    nx, ny, nz, h = 60, 57, 37, 10.0
     vel = synth.layered_model(nx, ny, nz, h, seed=8)
     slow = (1.0 / vel).astype(np.float32)
     ctx.set_model(slow, h)
@@ -150,13 +153,13 @@ def test_model_update_bit_exact(ctx, smooth):
     nt = ctx.tomo_set_grid(d)
     rng = np.random.default_rng(1)
     xdm = (rng.normal(0, 1e-5, nt[0] * nt[1] * nt[2])).astype(np.float32)
-    dm, velo = ctx.model_update(xdm, smooth=smooth, samples=5, stdv=2.0)
+    dm, velo = ctx.model_update(xdm, smooth=smooth, samples=samples, stdv=2.0)
     s_gpu = ctx.get_model()
     s_cpu = slow.copy()
     dm_c = np.zeros_like(slow)
     vel_c = np.zeros_like(slow)
     orc.oracle().or_model_update(P(s_cpu), P(xdm), nx, ny, nz, h, h, h, *nt, d, d, d,
-                                 int(smooth), 5, 2.0, P(dm_c), P(vel_c))
+                                 int(smooth), samples, 2.0, P(dm_c), P(vel_c))
     assert np.array_equal(dm, dm_c)
     assert np.array_equal(velo, vel_c)
     assert np.array_equal(s_gpu, s_cpu)
