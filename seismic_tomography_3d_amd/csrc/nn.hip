@@ -683,7 +683,10 @@ void conv_forward_any(hipStream_t st, const void *X0, int C0, bool up0, const vo
                ST_ERR_ARG, "conv shape out of range");
     ST_REQUIRE(C1 == 0 || C0 % 4 == 0, ST_ERR_ARG, "concatenated inputs need C0 % 4 == 0");
     const int cinp = conv_cinp(cin), coutp = conv_coutp(cout);
-    const int bx = cinp <= 48 ? 8 : 4;
+    // bricks: 16 wide for 16 output channels from at most 16 inputs (four
+    // m-tiles per wave share each weight fragment), 8 up to 48 inputs (a
+    // 16-wide 48-channel halo fills 67 KB of LDS and ran slower), else 4
+    const int bx = (coutp == 16 && cinp <= 16) ? 16 : (cinp <= 48 ? 8 : 4);
     const long long nwg = (long long)ns * ((e + bx - 1) / bx) * ((e + 3) / 4) * ((e + 3) / 4);
     ST_REQUIRE(nwg < (1LL << 31), ST_ERR_ARG, "conv grid too large");
     if (nwg == 0) return;
@@ -709,8 +712,8 @@ void conv_forward_any(hipStream_t st, const void *X0, int C0, bool up0, const vo
         ST_HIP(hipGetLastError());                                                                \
         return;                                                                                   \
     }
-    ST_CONV(3, 8, 1, 8)
-    ST_CONV(3, 16, 1, 8)
+    ST_CONV(3, 8, 1, 16)
+    ST_CONV(3, 16, 1, 16)
     ST_CONV(3, 48, 1, 8)
     ST_CONV(3, 16, 2, 8)
     ST_CONV(3, 32, 2, 8)
@@ -718,7 +721,7 @@ void conv_forward_any(hipStream_t st, const void *X0, int C0, bool up0, const vo
     ST_CONV(3, 32, 4, 8)
     ST_CONV(3, 64, 4, 4)
     ST_CONV(3, 128, 4, 4)
-    ST_CONV(1, 16, 1, 8)
+    ST_CONV(1, 16, 1, 16)
 #undef ST_CONV
 #undef ST_CONV_MODE
     ST_REQUIRE(false, ST_ERR_ARG, "no convolution kernel for this channel configuration");
