@@ -115,7 +115,7 @@ void gemm(hipStream_t st, const GemmArgs &g) {
 // its B fragment one 8-byte load of the fragment-ordered weights (a wave
 // reads 512 contiguous bytes; n padded to 16 NT, c to CINP, a zero tap
 // absorbing the K padding of CINP = 8).  The B fragments of a wave's next
-// tap group are loaded while the current one computes.
+// PD tap groups are in flight while the current one computes.
 // ---------------------------------------------------------------------------
 template <int KS, int CINP, int NT, int BX, bool VEC, bool IBF, bool OBF>
 __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, int C0, int up0,
@@ -166,7 +166,11 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
     // VEC: both inputs have a multiple of 4 channels (one 16-byte load per
     // item); otherwise the channels are read one by one.
     constexpr int C4 = CINP / 4;
-    constexpr int NI = NH * C4, SU = 8;
+    // BF16 halos of the 16-output-channel layers (few registers in use)
+    // are loaded in one batch where 24 items per thread cover them (two
+    // registers an item); the others in batches of 8
+    constexpr int NI = NH * C4, NPT = (NI + 255) / 256;
+    constexpr int SU = (IBF && NT == 1) ? (NPT < 24 ? NPT : 24) : 8;
     for (int base = 0; base < NI; base += 256 * SU) {
         float f[SU][4];      // FP32 inputs
         uint2 raw[SU];       // BF16 inputs, 4 channels as loaded
@@ -249,7 +253,12 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
 #pragma unroll
         for (int j = 0; j < NT; j++) acc[t][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    s4v bcur[GS][NT], bnxt[GS][NT];
+    // B fragments of the next PD tap groups in flight: a ring indexed at
+    // compile time (the group loop is unrolled PD times), so no register
+    // copy waits for a load issued in the same iteration
+    constexpr int PDR = 32 / (2 * GS * NT);
+    constexpr int PD = PDR < 2 ? 2 : (PDR > 8 ? 8 : PDR);
+    s4v bq[PD][GS][NT];
     auto load_b = [&](int g, s4v (&b)[GS][NT]) {
 #pragma unroll
         for (int k = 0; k < GS; k++)
@@ -259,29 +268,32 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
                     Wp + ((((long long)g * GS + k) * NT + j) * 64 + lane) * 4);
     };
     const int g0 = KSPLIT ? w : 0;
-    if (g0 < NG) load_b(g0, bcur);
-    for (int g = g0; g < NG; g += GSTEP) {
-        if (g + GSTEP < NG) load_b(g + GSTEP, bnxt);
-        int tap = tap_of(g);
-        if (tap >= TAPS) tap = 0;  // padding tap: its weights are zero
-        const int tx = tap % KS, ty = (tap / KS) % KS, tz = tap / (KS * KS);
-        const int toff = tx + HX * (ty + H * tz);
 #pragma unroll
-        for (int k = 0; k < GS; k++) {
-            s4v a[MT];
+    for (int d = 0; d < PD; d++)
+        if (g0 + d * GSTEP < NG) load_b(g0 + d * GSTEP, bq[d]);
+    for (int gb = g0; gb < NG; gb += PD * GSTEP) {
 #pragma unroll
-            for (int t = 0; t < MT; t++)
-                a[t] = *reinterpret_cast<const s4v *>(&halo[(hb[t] + toff) * CST + k * 16 + cq]);
+        for (int d = 0; d < PD; d++) {
+            const int g = gb + d * GSTEP;
+            if (g >= NG) break;
+            int tap = tap_of(g);
+            if (tap >= TAPS) tap = 0;  // padding tap: its weights are zero
+            const int tx = tap % KS, ty = (tap / KS) % KS, tz = tap / (KS * KS);
+            const int toff = tx + HX * (ty + H * tz);
 #pragma unroll
-            for (int t = 0; t < MT; t++)
+            for (int k = 0; k < GS; k++) {
+                s4v a[MT];
 #pragma unroll
-                for (int j = 0; j < NT; j++)
-                    acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bcur[k][j], acc[t][j], 0, 0, 0);
+                for (int t = 0; t < MT; t++)
+                    a[t] = *reinterpret_cast<const s4v *>(&halo[(hb[t] + toff) * CST + k * 16 + cq]);
+#pragma unroll
+                for (int t = 0; t < MT; t++)
+#pragma unroll
+                    for (int j = 0; j < NT; j++)
+                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bq[d][k][j], acc[t][j], 0, 0, 0);
+            }
+            if (g + PD * GSTEP < NG) load_b(g + PD * GSTEP, bq[d]);
         }
-#pragma unroll
-        for (int k = 0; k < GS; k++)
-#pragma unroll
-            for (int j = 0; j < NT; j++) bcur[k][j] = bnxt[k][j];
     }
 
     if constexpr (KSPLIT) {
