@@ -594,8 +594,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "shot-solves/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batched eikonal: 256 shots, 201x201x101, h=25 m, linear "
-                               "1500-4000 m/s + -10% anomaly, 400 surface receivers",
+        "config": {"workload": f"batched eikonal: {len(src)} shots, 201x201x101, h=25 m, linear "
+                               f"1500-4000 m/s + -10% anomaly, {len(rcv)} surface receivers",
                    "shots": len(src), "receivers": len(rcv), "block": B,
                    "batch": a.batch, "grid": params["grid"],
                    "max_inner": params["max_inner"], "warm_start": bool(warm),
