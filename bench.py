@@ -48,6 +48,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Host cores for the CPU baseline: every core this process may run on,
+    up to 16 -- the host share of one GPU on the GPU machines (whose
+    os.cpu_count() reports the whole host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_s=20.0):
     """CPU reference timing + FP64 error check (the checker lives here)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -71,7 +89,19 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
         te = np.abs(gpu_T_sample[k].astype(np.float64) - T64).max()
         errs[k] = (np.abs(pe).max(), float(np.sqrt(np.mean(pe ** 2))), te)
 
-    ths = [threading.Thread(target=check, args=(k, i)) for k, i in enumerate(sample_idx)]
+    # a pool of `threads` workers over the sample shots
+    todo = list(enumerate(sample_idx))
+    lock = threading.Lock()
+
+    def worker():
+        while True:
+            with lock:
+                if not todo:
+                    return
+                k, i = todo.pop()
+            check(k, i)
+
+    ths = [threading.Thread(target=worker) for _ in range(min(threads, len(sample_idx)))]
     t0 = time.time()
     for t in ths:
         t.start()
@@ -88,38 +118,41 @@ def cpu_baseline_leg(case, gpu_picks, gpu_T_sample, sample_idx, threads, budget_
         "fp64_oracle_s_per_shot": t_fp64 * min(threads, len(sample_idx)) / len(sample_idx),
     }
 
-    # CPU reference: the reference's own Accurate_FSM when built, else the port
+    # CPU reference: the reference's own Accurate_FSM when built, else the
+    # port; one shot per thread, two shots per thread per run, median of
+    # three runs (the sample is bounded: a few seconds of wall time)
     kind = "reference" if orc.ref_available() else "port"
-    n = threads
-    picks = np.zeros((n, nr), np.float32)
+
+    def run(sub, nthreads):
+        out = np.zeros((len(sub), nr), np.float32)
+        t = time.time()
+        if kind == "reference":
+            orc.ref().ref_fsm_threads(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), len(sub),
+                                      orc.P(rx), orc.P(ry), orc.P(rz), nr, orc.P(out), nthreads)
+        else:
+            orc.oracle().or_fsm_batch(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), len(sub),
+                                      orc.P(rx), orc.P(ry), orc.P(rz), nr, orc.P(out), nthreads)
+        return time.time() - t
+
+    n = 2 * threads
     sub = np.ascontiguousarray(src[:n])
-    t0 = time.time()
-    if kind == "reference":
-        orc.ref().ref_fsm_threads(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), n, orc.P(rx),
-                                  orc.P(ry), orc.P(rz), nr, orc.P(picks), threads)
-    else:
-        orc.oracle().or_fsm_batch(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub), n, orc.P(rx),
-                                  orc.P(ry), orc.P(rz), nr, orc.P(picks), threads)
-    dt = time.time() - t0
-    # the same on one core, one shot
-    p1 = np.zeros((1, nr), np.float32)
-    t1 = time.time()
-    if kind == "reference":
-        orc.ref().ref_fsm_threads(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub[:1]), 1, orc.P(rx),
-                                  orc.P(ry), orc.P(rz), nr, orc.P(p1), 1)
-    else:
-        orc.oracle().or_fsm_batch(orc.P(s), nx, ny, nz, h, h, h, orc.P(sub[:1]), 1, orc.P(rx),
-                                  orc.P(ry), orc.P(rz), nr, orc.P(p1), 1)
-    dt1 = time.time() - t1
+    walls = [run(sub, threads) for _ in range(3)]
+    dt = float(np.median(walls))
+    one = [run(np.ascontiguousarray(src[k:k + 1]), 1) for k in range(3)]
+    dt1 = float(np.median(one))
     base = {
         "value": n / dt,
         "unit": "shot-solves/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"{n} of the {len(src)} shots (one per thread), Accurate_FSM "
+        "sample": f"{n} of the {len(src)} shots ({n // threads} per thread), Accurate_FSM "
                   f"(Noble-Gesret-Belayouni, the reference's default CPU eikonal) + picks, "
-                  f"201x201x101, {dt:.1f} s wall",
-        "one_core": {"value": 1.0 / dt1, "unit": "shot-solves/s", "sample": "shot 0, 1 thread"},
+                  f"201x201x101; median of 3 runs: {', '.join(f'{w:.2f}' for w in walls)} s wall",
+        "cpu_model": cpu_model(),
+        "host_cpus": {"affinity": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+                      "used": threads},
+        "one_core": {"value": 1.0 / dt1, "unit": "shot-solves/s",
+                     "sample": "shots 0-2 one at a time on 1 thread, median"},
     }
     return base, err
 
@@ -461,7 +494,8 @@ def main():
     ap.add_argument("--warm", default="auto", choices=["auto", "on", "off"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--check-shots", type=int, default=8)
+    ap.add_argument("--check-shots", type=int, default=256,
+                    help="shots checked against the FP64 oracle after the timed steps")
     ap.add_argument("--band", type=float, default=0.0,
                     help="warm-start ordering band in s (0 = tuned)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot-at-a-time leg")
@@ -620,7 +654,7 @@ def main():
         }
 
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = a.cpu_threads or host_threads()
         ks = max(1, min(a.check_shots, len(src)))
         sample = np.linspace(0, len(src) - 1, ks).round().astype(int)
         _, Ts, _ = ctx.fim_solve(src[sample], rcv, return_T=True, **opts)

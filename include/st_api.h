@@ -122,7 +122,7 @@ int st_fim_solve_warm(st_ctx *ctx, const float *src_xyz, int nshot,
 
 /* The reference's CPU eikonal, Accurate_FSM (Noble-Gesret-Belayouni
  * operators: accurate_FSM.cpp:577-650 eikonal_equation with
- * write_first_arrival :528-556), on the device for all shots of the call,
+ * write_first_arrival :541-575), on the device for all shots of the call,
  * bit-identical to the CPU code on the same model (isotropic or not).
  * Replaces Accurate_FSM::eikonal_equation + write_first_arrival per shot
  * (selected by eikonal_type, eikonal_main.cpp:15-22).  picks[nshot x nrcv];
@@ -158,6 +158,24 @@ int st_tomo_dims(st_ctx *ctx, int *nx_tomo, int *ny_tomo, int *nz_tomo);
 int st_forward(st_ctx *ctx, const float *src_xyz, int nshot,
                const float *rcv_xyz, int nrcv, float *dcal, float *illum,
                const st_fim_opts *opts, st_fim_stats *stats);
+
+/* Least_squares::set_parameters' choice of the forward eikonal
+ * (least_squares.cpp:85-104): 0 Podvin & Lecomte (Classic), 1 block FIM,
+ * 2 Accurate_FSM; any other value selects Accurate_FSM, the switch's default
+ * branch.  st_forward then solves with that method and traces the rays on
+ * its travel times (Accurate_FSM and Podvin bit-identical to the CPU code,
+ * so picks and G equal the reference's).  A new context uses 1.  The
+ * st_fim_opts of st_forward apply to the block FIM only (a warm start with
+ * another method is ST_ERR_ARG). */
+int st_tomo_set_eikonal(st_ctx *ctx, int eikonal_type);
+
+/* As st_forward, and T_out (nshot*nx*ny*nz, z fastest) receives each shot's
+ * travel-time volume, the one its rays were traced on: the reference's
+ * eikonal->travel_time that expand_fdm pads for ray_tracing
+ * (least_squares.cpp:175-181). */
+int st_forward_times(st_ctx *ctx, const float *src_xyz, int nshot,
+                     const float *rcv_xyz, int nrcv, float *dcal, float *illum,
+                     float *T_out, const st_fim_opts *opts, st_fim_stats *stats);
 
 /* Ray tracing on a caller volume T (z fastest, nx*ny*nz, the current model
  * grid) for one source; rows are appended to G.  st_csr_reset clears G. */

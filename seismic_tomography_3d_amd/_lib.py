@@ -74,6 +74,10 @@ SIGNATURES = [
     ("st_forward", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
                                   ctypes.c_int, c_float_p, c_float_p, ctypes.POINTER(FimOpts),
                                   ctypes.POINTER(FimStats)]),
+    ("st_tomo_set_eikonal", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("st_forward_times", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                        ctypes.c_int, c_float_p, c_float_p, c_float_p,
+                                        ctypes.POINTER(FimOpts), ctypes.POINTER(FimStats)]),
     ("st_csr_reset", ctypes.c_int, [ctypes.c_void_p]),
     ("st_raytrace_volume", ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p,
                                           ctypes.c_int, c_float_p]),

@@ -217,14 +217,27 @@ class Context:
         self._check(self.lib.st_tomo_dims(self.h, *[ctypes.byref(v) for v in n]))
         return tuple(v.value for v in n)  # (nx_tomo, ny_tomo, nz_tomo)
 
-    def forward(self, src, rcv, illum=False, **opts):
-        """FIM + picks + ray tracing into the device G.  Returns (dcal, illum, stats)."""
+    def tomo_set_eikonal(self, eikonal_type):
+        """The forward pass's eikonal: 0 Podvin, 1 block FIM, 2 Accurate_FSM
+        (other values: Accurate_FSM, as the reference's default branch)."""
+        self._check(self.lib.st_tomo_set_eikonal(self.h, int(eikonal_type)))
+
+    def forward(self, src, rcv, illum=False, return_T=False, **opts):
+        """Eikonal + picks + ray tracing into the device G.  Returns (dcal,
+        illum, stats), and the travel times T[nshot, ny, nx, nz] the rays were
+        traced on as a fourth item when return_T."""
         src = _f32(src, (-1, 3))
         rcv = _f32(rcv, (-1, 3))
         dcal = np.empty((src.shape[0], rcv.shape[0]), np.float32)
         il = np.empty(self.shape, np.float32) if illum else None
         o = self.options(**opts)
         st = FimStats()
+        if return_T:
+            T = np.empty((src.shape[0],) + self.shape, np.float32)
+            self._check(self.lib.st_forward_times(self.h, _ptr(src), src.shape[0], _ptr(rcv),
+                                                  rcv.shape[0], _ptr(dcal), _ptr(il), _ptr(T),
+                                                  ctypes.byref(o), ctypes.byref(st)))
+            return dcal, il, st.as_dict(), T
         self._check(self.lib.st_forward(self.h, _ptr(src), src.shape[0], _ptr(rcv), rcv.shape[0],
                                         _ptr(dcal), _ptr(il), ctypes.byref(o), ctypes.byref(st)))
         return dcal, il, st.as_dict()

@@ -56,6 +56,8 @@ void tomo_trace_blk(st_ctx *ctx, const float *T, long long vol, int B, int nbx, 
                     const float *dsrc, int nshot, const float *drcv, int nr, bool with_illum);
 void tomo_trace_zxy(st_ctx *ctx, const float *dT, const float *dsrc, const float *drcv, int nr,
                     bool with_illum);
+void tomo_trace_pad(st_ctx *ctx, const float *T, long long shot_stride, int cell_stride,
+                    const float *dsrc, int nshot, const float *drcv, int nr, bool with_illum);
 void tomo_get_illum(st_ctx *ctx, float *out);
 void tomo_csr(st_ctx *ctx, long long *rows, long long *nnz, const long long **rp, const int **c,
               const float **v);
@@ -96,6 +98,60 @@ void check_points(const float *p, int n, const st_ctx *ctx, const char *what) {
         ST_REQUIRE(ok, ST_ERR_ARG,
                    std::string(what) + " " + std::to_string(i) + " lies outside the model");
     }
+}
+
+// Smallest and largest slowness as IEEE bit patterns: for finite positive
+// floats the unsigned order is the numeric order; +0 is the smallest
+// pattern, while infinities, NaNs and negative values (sign bit set) are at
+// or above 0x7F800000, so two integer extremes decide validity too.
+__global__ void slowness_range(const unsigned *__restrict__ s, long long n,
+                               unsigned *__restrict__ out) {
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const unsigned v = s[i];
+        lo = min(lo, v);
+        hi = max(hi, v);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, (unsigned)__shfl_xor((int)lo, o));
+        hi = max(hi, (unsigned)__shfl_xor((int)hi, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&out[0], lo);
+        atomicMax(&out[1], hi);
+    }
+}
+
+__global__ void range_init(unsigned *out) {
+    out[0] = 0xFFFFFFFFu;
+    out[1] = 0u;
+}
+
+void model_changed(st_ctx *ctx) {
+    ctx->blk_valid[0] = ctx->blk_valid[1] = false;
+    ctx->fsm_valid = false;
+    ctx->range_valid = false;
+}
+
+void ensure_range(st_ctx *ctx) {
+    if (ctx->range_valid) return;
+    const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
+    ctx->range.reserve(64);
+    unsigned *d = ctx->range.as<unsigned>();
+    hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, ctx->stream, d);
+    const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(slowness_range, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->slow_zxy.as<unsigned>(), n, d);
+    ST_HIP(hipGetLastError());
+    unsigned *h = static_cast<unsigned *>(ctx->host_pinned) + 240;
+    ST_HIP(hipMemcpyAsync(h, d, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    ST_REQUIRE(h[0] > 0u && h[1] < 0x7F800000u, ST_ERR_ARG,
+               "slowness must be finite and positive everywhere (after a model update too)");
+    ctx->smin = __builtin_bit_cast(float, h[0]);
+    ctx->smax = __builtin_bit_cast(float, h[1]);
+    ctx->range_valid = true;
 }
 
 }  // namespace st
@@ -155,7 +211,7 @@ int st_ctx_destroy(st_ctx *ctx) {
     wave_release(ctx);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    for (auto *b : {&ctx->slow_zxy, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
+    for (auto *b : {&ctx->slow_zxy, &ctx->range, &ctx->slow_blk[0], &ctx->slow_blk[1], &ctx->T, &ctx->flags,
                     &ctx->list[0], &ctx->list[1], &ctx->counters, &ctx->wstats, &ctx->fsm_S,
                     &ctx->fsm_sid, &ctx->pod_T, &ctx->pod_stamp, &ctx->pod_list,
                     &ctx->pod_newv, &ctx->pod_counts, &ctx->pod_nit, &ctx->pod_temp,
@@ -189,11 +245,11 @@ int st_set_model(st_ctx *ctx, const float *slowness, int nx, int ny, int nz, flo
         ctx->slow_zxy.reserve(sizeof(float) * n);
         ST_HIP(hipMemcpyAsync(ctx->slow_zxy.ptr, slowness, sizeof(float) * n,
                               hipMemcpyHostToDevice, ctx->stream));
-        ctx->blk_valid[0] = ctx->blk_valid[1] = false;
-        ctx->fsm_valid = false;
+        model_changed(ctx);
         ctx->has_model = true;
         ctx->smin = *std::min_element(slowness, slowness + n);
         ctx->smax = *std::max_element(slowness, slowness + n);
+        ctx->range_valid = true;
         ST_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -209,6 +265,78 @@ int st_get_model(st_ctx *ctx, float *slowness) {
                               hipMemcpyDeviceToHost, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
     });
+}
+
+// Sources and receivers to the device (src: nshot triples, rcv: nrcv).
+static void upload_points(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
+                          int nrcv) {
+    ctx->src.reserve(sizeof(float) * 3 * std::max(nshot, 1));
+    ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
+    ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
+    ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot, hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (nrcv)
+        ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
+                              hipMemcpyHostToDevice, ctx->stream));
+}
+
+// The sweeping solver's padded slowness (rebuilt after a model change) and
+// travel-time buffer for batches of shots; returns the batch size (batch <= 0:
+// 256, halved until a batch fits in half of the free device memory).
+static int fsm_setup(st_ctx *ctx, int nshot, int batch, bool with_T_out) {
+    const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+    const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+    if (!ctx->fsm_valid) {
+        ctx->fsm_S.reserve(sizeof(float) * nB);
+        fsm_pad(ctx, ctx->fsm_S.as<float>());
+        ctx->fsm_valid = true;
+    }
+    int b = batch > 0 ? std::min({batch, nshot, 1023}) : std::min(nshot, 256);
+    size_t free_b = 0, total_b = 0;
+    ST_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_shot = sizeof(float) * nB + (with_T_out ? sizeof(float) * npts : 0);
+    const size_t have = free_b + ctx->T.bytes + ctx->scratch.bytes;
+    while (b > 1 && per_shot * (size_t)fsm_lanes(b) > have / 2) b /= 2;
+    ctx->T.reserve(sizeof(float) * nB * fsm_lanes(b));
+    if (with_T_out) ctx->scratch.reserve(sizeof(float) * npts * b);
+    return b;
+}
+
+// Podvin's padded slowness (the sweeping solver's one-cell replicated pad)
+// and per-shot buffers; returns the batch size (batch <= 0: 16).
+static int podvin_setup(st_ctx *ctx, int nshot, int batch, bool with_T_out) {
+    const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+    const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+    if (!ctx->fsm_valid) {
+        ctx->fsm_S.reserve(sizeof(float) * nB);
+        fsm_pad(ctx, ctx->fsm_S.as<float>());
+        ctx->fsm_valid = true;
+    }
+    int b = batch > 0 ? std::min(batch, nshot) : std::min(nshot, 16);
+    // list items and counts are 32-bit: shots x padded cells below 2^31
+    while (b > 1 && (long long)b * nB >= (1LL << 31)) b /= 2;
+    size_t free_b = 0, total_b = 0;
+    ST_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_shot = podvin_bytes_per_shot(ctx) + (with_T_out ? sizeof(float) * npts : 0);
+    const size_t have = free_b + ctx->pod_T.bytes + ctx->pod_stamp.bytes + ctx->pod_list.bytes +
+                        ctx->pod_newv.bytes + ctx->scratch.bytes;
+    while (b > 1 && per_shot * (size_t)b > have / 2) b /= 2;
+    ctx->pod_T.reserve(sizeof(float) * nB * b);
+    ctx->pod_stamp.reserve(sizeof(int) * nB * b);
+    ctx->pod_list.reserve(sizeof(int) * nB * b);
+    ctx->pod_newv.reserve(sizeof(float) * nB * b);
+    ctx->pod_counts.reserve(sizeof(int) * 64);
+    ctx->pod_nit.reserve(sizeof(int) * b);
+    if (with_T_out) ctx->scratch.reserve(sizeof(float) * npts * b);
+    return b;
+}
+
+// Podvin over shots b0 .. b0 + nb - 1 of the uploaded sources into pod_T.
+static double podvin_run(st_ctx *ctx, int b0, int nb, const float *src_xyz) {
+    return podvin_batch(ctx, ctx->src.as<float>() + 3LL * b0, src_xyz + 3LL * b0, nb,
+                        ctx->fsm_S.as<float>(), ctx->pod_T.as<float>(), ctx->pod_stamp.as<int>(),
+                        ctx->pod_list.as<int>(), ctx->pod_newv.as<float>(),
+                        ctx->pod_counts.as<int>(), ctx->pod_nit.as<int>(), ctx->pod_temp);
 }
 
 static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
@@ -228,6 +356,8 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
     ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
     ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
     const bool warm = (o.warm == 1) || T_pred;
+    set_device(ctx);
+    ensure_range(ctx);
     if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
     if (o.band < 0.f || !warm) o.band = 1e30f;
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -386,29 +516,10 @@ int st_fsm_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_
         check_points(src_xyz, nshot, ctx, "source");
         check_points(rcv_xyz, nrcv, ctx, "receiver");
         set_device(ctx);
-        const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+        ensure_range(ctx);
         const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
-        if (!ctx->fsm_valid) {
-            ctx->fsm_S.reserve(sizeof(float) * nB);
-            fsm_pad(ctx, ctx->fsm_S.as<float>());
-            ctx->fsm_valid = true;
-        }
-        int b = batch > 0 ? std::min({batch, nshot, 1023}) : std::min(nshot, 256);
-        size_t free_b = 0, total_b = 0;
-        ST_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t per_shot = sizeof(float) * nB + (T_out ? sizeof(float) * npts : 0);
-        const size_t have = free_b + ctx->T.bytes + ctx->scratch.bytes;
-        while (b > 1 && per_shot * (size_t)fsm_lanes(b) > have / 2) b /= 2;
-        ctx->T.reserve(sizeof(float) * nB * fsm_lanes(b));
-        if (T_out) ctx->scratch.reserve(sizeof(float) * npts * b);
-        ctx->src.reserve(sizeof(float) * 3 * nshot);
-        ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
-        ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
-        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
-                              hipMemcpyHostToDevice, ctx->stream));
-        if (nrcv)
-            ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
-                                  hipMemcpyHostToDevice, ctx->stream));
+        const int b = fsm_setup(ctx, nshot, batch, T_out != nullptr);
+        upload_points(ctx, src_xyz, nshot, rcv_xyz, nrcv);
         double ms = 0.0;
         for (int b0 = 0; b0 < nshot; b0 += b) {
             const int nb = std::min(b, nshot - b0);
@@ -446,45 +557,14 @@ int st_podvin_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *r
         check_points(src_xyz, nshot, ctx, "source");
         check_points(rcv_xyz, nrcv, ctx, "receiver");
         set_device(ctx);
-        const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+        ensure_range(ctx);
         const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
-        if (!ctx->fsm_valid) {   // the same one-cell replicated padding as the sweeping solver
-            ctx->fsm_S.reserve(sizeof(float) * nB);
-            fsm_pad(ctx, ctx->fsm_S.as<float>());
-            ctx->fsm_valid = true;
-        }
-        int b = batch > 0 ? std::min(batch, nshot) : std::min(nshot, 16);
-        // list items and counts are 32-bit: shots x padded cells below 2^31
-        while (b > 1 && (long long)b * nB >= (1LL << 31)) b /= 2;
-        size_t free_b = 0, total_b = 0;
-        ST_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t per_shot = podvin_bytes_per_shot(ctx) + (T_out ? sizeof(float) * npts : 0);
-        const size_t have = free_b + ctx->pod_T.bytes + ctx->pod_stamp.bytes +
-                            ctx->pod_list.bytes + ctx->pod_newv.bytes + ctx->scratch.bytes;
-        while (b > 1 && per_shot * (size_t)b > have / 2) b /= 2;
-        ctx->pod_T.reserve(sizeof(float) * nB * b);
-        ctx->pod_stamp.reserve(sizeof(int) * nB * b);
-        ctx->pod_list.reserve(sizeof(int) * nB * b);
-        ctx->pod_newv.reserve(sizeof(float) * nB * b);
-        ctx->pod_counts.reserve(sizeof(int) * 64);
-        ctx->pod_nit.reserve(sizeof(int) * b);
-        if (T_out) ctx->scratch.reserve(sizeof(float) * npts * b);
-        ctx->src.reserve(sizeof(float) * 3 * nshot);
-        ctx->rcv.reserve(sizeof(float) * 3 * std::max(nrcv, 1));
-        ctx->picks.reserve(sizeof(float) * std::max<long long>(1, (long long)nshot * nrcv));
-        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
-                              hipMemcpyHostToDevice, ctx->stream));
-        if (nrcv)
-            ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
-                                  hipMemcpyHostToDevice, ctx->stream));
+        const int b = podvin_setup(ctx, nshot, batch, T_out != nullptr);
+        upload_points(ctx, src_xyz, nshot, rcv_xyz, nrcv);
         double ms = 0.0;
         for (int b0 = 0; b0 < nshot; b0 += b) {
             const int nb = std::min(b, nshot - b0);
-            ms += podvin_batch(ctx, ctx->src.as<float>() + 3LL * b0, src_xyz + 3LL * b0, nb,
-                               ctx->fsm_S.as<float>(), ctx->pod_T.as<float>(),
-                               ctx->pod_stamp.as<int>(), ctx->pod_list.as<int>(),
-                               ctx->pod_newv.as<float>(), ctx->pod_counts.as<int>(),
-                               ctx->pod_nit.as<int>(), ctx->pod_temp);
+            ms += podvin_run(ctx, b0, nb, src_xyz);
             if (nrcv)
                 podvin_picks(ctx, ctx->pod_T.as<float>(), nb, ctx->rcv.as<float>(), nrcv,
                              ctx->picks.as<float>() + (long long)b0 * nrcv);
@@ -568,18 +648,29 @@ int st_tomo_dims(st_ctx *ctx, int *nxt, int *nyt, int *nzt) {
     });
 }
 
-int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz, int nrcv,
-               float *dcal, float *illum, const st_fim_opts *opts, st_fim_stats *stats) {
+int st_tomo_set_eikonal(st_ctx *ctx, int eikonal_type) {
     if (!ctx) return ST_ERR_ARG;
     return guarded(ctx, [&] {
-        int n[3], ms, mr;
-        float d[3];
-        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
-        ST_REQUIRE(tomo_grid(ctx, n, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
-        ST_REQUIRE(nshot > 0 && nrcv > 0 && src_xyz && rcv_xyz && dcal, ST_ERR_ARG,
-                   "forward needs sources, receivers and a dcal buffer");
-        st_fim_opts o;
-        if (opts) o = *opts; else st_fim_default_opts(&o);
+        // least_squares.cpp:85-104: 0 Classic, 1 Block_FIM, 2 Accurate_FSM,
+        // any other value Accurate_FSM (the switch's default branch)
+        ctx->eikonal_type = (eikonal_type == 0 || eikonal_type == 1) ? eikonal_type : 2;
+    });
+}
+
+static void forward_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
+                         int nrcv, float *dcal, float *illum, float *T_out,
+                         const st_fim_opts *opts, st_fim_stats *stats) {
+    const auto t_start = std::chrono::steady_clock::now();
+    int n[3], ms, mr;
+    float d[3];
+    ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+    ST_REQUIRE(tomo_grid(ctx, n, d, &ms, &mr), ST_ERR_STATE, "st_tomo_set_grid first");
+    ST_REQUIRE(nshot > 0 && nrcv > 0 && src_xyz && rcv_xyz && dcal, ST_ERR_ARG,
+               "forward needs sources, receivers and a dcal buffer");
+    st_fim_opts o;
+    if (opts) o = *opts; else st_fim_default_opts(&o);
+    const int type = ctx->eikonal_type;
+    if (type == 1) {
         ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
         if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 24 : 16;
         if (o.eps < 0.f) o.eps = kFimEps;
@@ -589,32 +680,40 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
         ST_REQUIRE(o.warm == 0 || o.warm_values == 0, ST_ERR_ARG,
                    "forward warm starts order visits only (warm_values = 0)");
         ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
+    } else {
+        ST_REQUIRE(o.warm == 0, ST_ERR_ARG, "warm starts apply to the block FIM (eikonal_type 1)");
+    }
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    check_points(src_xyz, nshot, ctx, "source");
+    check_points(rcv_xyz, nrcv, ctx, "receiver");
+    set_device(ctx);
+    ensure_range(ctx);
+    const long long npts = (long long)ctx->nx * ctx->ny * ctx->nz;
+    upload_points(ctx, src_xyz, nshot, rcv_xyz, nrcv);
+    tomo_reset(ctx, true);
+    // one batch: solve, pick, trace its rays; the travel times to T_out
+    auto to_host = [&](int b0, int nb) {
+        ST_HIP(hipMemcpyAsync(T_out + (long long)b0 * npts, ctx->scratch.ptr,
+                              sizeof(float) * npts * nb, hipMemcpyDeviceToHost, ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+    };
+    const float *drcv = ctx->rcv.as<float>();
+    if (type == 1) {
         const bool warm = o.warm == 1;
         if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
         if (o.band < 0.f || !warm) o.band = 1e30f;
-        if (stats) std::memset(stats, 0, sizeof(*stats));
-        check_points(src_xyz, nshot, ctx, "source");
-        check_points(rcv_xyz, nrcv, ctx, "receiver");
-        set_device(ctx);
         const int B = o.block;
         ensure_block_model(ctx, B);
         const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
-        int batch = std::max(1, std::min({o.batch, nshot, 1023}));
+        const int batch = std::max(1, std::min({o.batch, nshot, 1023}));
         ctx->T.reserve(sizeof(float) * g.vol * batch);
         ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
         ctx->list[0].reserve(fim_list_bytes(g.nblk, batch));
         ctx->list[1].reserve(fim_list_bytes(g.nblk, batch));
-        ctx->src.reserve(sizeof(float) * 3 * nshot);
-        ctx->rcv.reserve(sizeof(float) * 3 * nrcv);
-        ctx->picks.reserve(sizeof(float) * (long long)nshot * nrcv);
         if (warm) ctx->keys.reserve(sizeof(float) * g.nblk * batch);
-        ST_HIP(hipMemcpyAsync(ctx->src.ptr, src_xyz, sizeof(float) * 3 * nshot,
-                              hipMemcpyHostToDevice, ctx->stream));
-        ST_HIP(hipMemcpyAsync(ctx->rcv.ptr, rcv_xyz, sizeof(float) * 3 * nrcv,
-                              hipMemcpyHostToDevice, ctx->stream));
+        if (T_out) ctx->scratch.reserve(sizeof(float) * npts * batch);
         const float r_cold = o.cold_radius > 0.f ? o.cold_radius
                                                  : (warm ? surrogate_cold_radius(ctx) : 0.f);
-        tomo_reset(ctx, true);
         for (int b0 = 0; b0 < nshot; b0 += batch) {
             const int nb = std::min(batch, nshot - b0);
             const float *dsrc = ctx->src.as<float>() + 3LL * b0;
@@ -626,16 +725,74 @@ int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xy
                 keys = ctx->keys.as<float>();
             }
             fim_batch(ctx, dsrc, nb, o, stats, warm, keys);
-            launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, ctx->rcv.as<float>(), nrcv,
+            launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, drcv, nrcv,
                              ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx, ctx->ny,
                              ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
-            tomo_trace_blk(ctx, ctx->T.as<float>(), g.vol, B, g.nbx, g.nby, dsrc, nb,
-                           ctx->rcv.as<float>(), nrcv, true);
+            tomo_trace_blk(ctx, ctx->T.as<float>(), g.vol, B, g.nbx, g.nby, dsrc, nb, drcv, nrcv,
+                           true);
+            if (T_out) {
+                launch_extract(ctx, ctx->scratch.as<float>(), nb, B);
+                to_host(b0, nb);
+            }
         }
-        ST_HIP(hipMemcpyAsync(dcal, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
-                              hipMemcpyDeviceToHost, ctx->stream));
-        if (illum) tomo_get_illum(ctx, illum);
-        ST_HIP(hipStreamSynchronize(ctx->stream));
+    } else if (type == 2) {
+        // Accurate_FSM: the padded, shot-fastest volumes are traced in place
+        const int b = fsm_setup(ctx, nshot, o.batch, T_out != nullptr);
+        for (int b0 = 0; b0 < nshot; b0 += b) {
+            const int nb = std::min(b, nshot - b0);
+            const float *dsrc = ctx->src.as<float>() + 3LL * b0;
+            const double t = fsm_batch(ctx, dsrc, src_xyz + 3LL * b0, nb, ctx->fsm_S.as<float>());
+            if (stats) { stats->solve_ms += t; stats->batches += 1; }
+            fsm_picks(ctx, nb, drcv, nrcv, ctx->picks.as<float>() + (long long)b0 * nrcv);
+            tomo_trace_pad(ctx, ctx->T.as<float>(), 1, fsm_lanes(nb), dsrc, nb, drcv, nrcv, true);
+            if (T_out) {
+                fsm_crop_volumes(ctx, ctx->scratch.as<float>(), nb);
+                to_host(b0, nb);
+            }
+        }
+    } else {
+        // Podvin & Lecomte: shot-major padded volumes
+        const int b = podvin_setup(ctx, nshot, 0, T_out != nullptr);
+        const long long nB = (long long)(ctx->nx + 2) * (ctx->ny + 2) * (ctx->nz + 2);
+        for (int b0 = 0; b0 < nshot; b0 += b) {
+            const int nb = std::min(b, nshot - b0);
+            const float *dsrc = ctx->src.as<float>() + 3LL * b0;
+            const double t = podvin_run(ctx, b0, nb, src_xyz);
+            if (stats) { stats->solve_ms += t; stats->batches += 1; }
+            podvin_picks(ctx, ctx->pod_T.as<float>(), nb, drcv, nrcv,
+                         ctx->picks.as<float>() + (long long)b0 * nrcv);
+            tomo_trace_pad(ctx, ctx->pod_T.as<float>(), nB, 1, dsrc, nb, drcv, nrcv, true);
+            if (T_out) {
+                podvin_crop_volumes(ctx, ctx->pod_T.as<float>(), ctx->scratch.as<float>(), nb);
+                to_host(b0, nb);
+            }
+        }
+    }
+    ST_HIP(hipMemcpyAsync(dcal, ctx->picks.ptr, sizeof(float) * (long long)nshot * nrcv,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    if (illum) tomo_get_illum(ctx, illum);
+    ST_HIP(hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+                .count();
+}
+
+int st_forward(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz, int nrcv,
+               float *dcal, float *illum, const st_fim_opts *opts, st_fim_stats *stats) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        forward_impl(ctx, src_xyz, nshot, rcv_xyz, nrcv, dcal, illum, nullptr, opts, stats);
+    });
+}
+
+int st_forward_times(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
+                     int nrcv, float *dcal, float *illum, float *T_out, const st_fim_opts *opts,
+                     st_fim_stats *stats) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        ST_REQUIRE(T_out, ST_ERR_ARG, "null travel-time output");
+        forward_impl(ctx, src_xyz, nshot, rcv_xyz, nrcv, dcal, illum, T_out, opts, stats);
     });
 }
 

@@ -930,6 +930,10 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         mix((long long)(uintptr_t)A.list1); mix((long long)(uintptr_t)A.kmin);
         mix((long long)(uintptr_t)A.stats);
         mix(nshot); mix(B); mix(grid); mix(A.nblk); mix(A.max_inner);
+        // every geometry field the kernels read: two models with the same
+        // block count can differ in the pad-to-4 extent or the block grid
+        mix(A.nbx); mix(A.nby); mix(A.nbz); mix(A.n4x); mix(A.n4y); mix(A.n4z);
+        mix(A.vol); mix(A.cap); mix(A.nshot);
         mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
         mix((long long)__builtin_bit_cast(unsigned, A.h));
         mix((long long)(uintptr_t)A.src); mix(hyp ? 1 : 0);

@@ -106,6 +106,17 @@ __host__ __device__ inline long long blk_addr(int x, int y, int z, int B, int nb
 
 }  // namespace st
 
+struct st_ctx;
+namespace st {
+// The device slowness (slow_zxy) changed: a new model, a model update or the
+// training-set generator's temporary models.  Drops every copy derived from
+// it (block layouts, the sweeping solvers' padded copy, the slowness range).
+void model_changed(st_ctx *ctx);
+// Recomputes smin/smax on the device when they are stale and requires a
+// finite, positive slowness everywhere (ST_ERR_ARG otherwise).
+void ensure_range(st_ctx *ctx);
+}  // namespace st
+
 // The context.
 struct st_ctx {
     int device = 0;
@@ -117,7 +128,9 @@ struct st_ctx {
     int nx = 0, ny = 0, nz = 0;
     float dx = 0, dy = 0, dz = 0;
     float smin = 0, smax = 0;          // slowness range of the model
+    bool range_valid = false;          // smin/smax describe slow_zxy
     st::DevBuf slow_zxy;               // slowness, z fastest (for updates)
+    st::DevBuf range;                  // device min/max of the slowness bits
     st::DevBuf slow_blk[2];            // block layouts for B = 4, 8
     bool blk_valid[2] = {false, false};
 
@@ -139,6 +152,10 @@ struct st_ctx {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     long long graph_key = -1;
+
+    // the eikonal solver of the tomography's forward pass (eikonal_type of
+    // Least_squares::set_parameters: 0 Podvin, 1 block FIM, 2 Accurate_FSM)
+    int eikonal_type = 1;
 
     // opaque extension state (tomography, surrogate, acoustic modelling)
     void *tomo = nullptr;

@@ -667,6 +667,7 @@ int st_surrogate_make_data(st_ctx *ctx, int n_models, int spm, float zmax_src, u
         ST_HIP(hipMemcpyAsync(saved.ptr, ctx->slow_zxy.ptr, sizeof(float) * npts,
                               hipMemcpyDeviceToDevice, ctx->stream));
         const float smin_saved = ctx->smin, smax_saved = ctx->smax;
+        const bool range_saved = ctx->range_valid;
         st_fim_opts o;
         st_fim_default_opts(&o);
         o.block = 8;
@@ -710,7 +711,7 @@ int st_surrogate_make_data(st_ctx *ctx, int n_models, int spm, float zmax_src, u
             hipLaunchKernelGGL(synth_model, dim3(nblocks(npts)), dim3(256), 0, ctx->stream, P,
                                ctx->slow_zxy.as<float>(), ctx->nx, ctx->ny, ctx->nz, ctx->dx,
                                ctx->dy, ctx->dz);
-            ctx->blk_valid[0] = ctx->blk_valid[1] = false;
+            model_changed(ctx);
             ctx->smax = 1.f / 1400.f;
             ensure_block_model(ctx, o.block);
             for (int k = 0; k < spm; k++) {
@@ -735,9 +736,10 @@ int st_surrogate_make_data(st_ctx *ctx, int n_models, int spm, float zmax_src, u
                               hipMemcpyDeviceToDevice, ctx->stream));
         ST_HIP(hipStreamSynchronize(ctx->stream));
         saved.release();
+        model_changed(ctx);
         ctx->smin = smin_saved;
         ctx->smax = smax_saved;
-        ctx->blk_valid[0] = ctx->blk_valid[1] = false;
+        ctx->range_valid = range_saved;
     });
 }
 

@@ -101,6 +101,16 @@ struct ZxyLay {
         return z + (long long)x * nz + (long long)y * nx * nz;
     }
 };
+// the sweeping solvers' volumes, padded by one cell on every side (the
+// reference's expand_fdm extent); cells are cs floats apart (Accurate_FSM
+// keeps the shots of a batch fastest, cs = its shot lanes; Podvin cs = 1)
+struct PadLay {
+    int nxx, nzz, cs;
+    __device__ long long operator()(int x, int y, int z) const {
+        return ((long long)(z + 1) + (long long)(x + 1) * nzz + (long long)(y + 1) * nxx * nzz) *
+               cs;
+    }
+};
 // solver block layout for a power-of-two block edge 1 << lb and in-volume
 // (non-negative) coordinates: shifts and masks instead of the three integer
 // divisions of blk_addr, and 32-bit offsets (the caller checks that one
@@ -655,7 +665,12 @@ __global__ void zero_vec(float *__restrict__ dst, long long n) {
 void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda) {
     TomoState &t = tomo(ctx);
     const long long n_data = t.rows;
-    ST_REQUIRE(order >= 0 && order < n_model, ST_ERR_ARG, "bad Tikhonov order");
+    // the stencil of order k holds the binomial coefficients C(k, j) (up to
+    // C(16, 8) = 12870 at the cap, exact in int and in float); the
+    // reference's parameter files use orders 0 to 2
+    constexpr int kMaxTkOrder = 16;
+    ST_REQUIRE(order >= 0 && order <= kMaxTkOrder && order < n_model, ST_ERR_ARG,
+               "Tikhonov order must lie in 0..16 and below the model size");
     const int nL = n_model - order;
     const long long N = n_data + nL;
     const long long annz = t.nnz + (long long)(order + 1) * nL;
@@ -1097,7 +1112,7 @@ void tomo_solve_update(st_ctx *ctx, int iters, int smooth, int samples, float st
                               ctx->stream));
     }
     ST_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->blk_valid[0] = ctx->blk_valid[1] = false;  // block layouts re-packed on next solve
+    model_changed(ctx);  // derived copies of the slowness are rebuilt on the next solve
 }
 
 void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, float stdv,
@@ -1132,7 +1147,13 @@ void tomo_model_update(st_ctx *ctx, const float *dxdm, int smooth, int samples, 
         ST_HIP(hipMemcpyAsync(vel_out, t.upd.ptr, sizeof(float) * n, hipMemcpyDeviceToHost,
                               ctx->stream));
     ST_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->blk_valid[0] = ctx->blk_valid[1] = false;  // block layouts re-packed on next solve
+    model_changed(ctx);  // derived copies of the slowness are rebuilt on the next solve
+}
+
+void tomo_trace_pad(st_ctx *ctx, const float *T, long long shot_stride, int cell_stride,
+                    const float *dsrc, int nshot, const float *drcv, int nr, bool with_illum) {
+    trace_append(ctx, T, shot_stride, PadLay{ctx->nx + 2, ctx->nz + 2, cell_stride}, dsrc, nshot,
+                 drcv, nr, with_illum);
 }
 
 void tomo_trace_zxy(st_ctx *ctx, const float *dT, const float *dsrc, const float *drcv, int nr,

@@ -18,11 +18,15 @@ void Least_squares::info_message() {
 }
 
 void Least_squares::set_parameters() {
+    // least_squares.cpp:85-104: 0 Classic, 1 Block_FIM, 2 Accurate_FSM,
+    // anything else Accurate_FSM
     const int type = std::stoi(catch_parameter("eikonal_type", parameters));
-    if (type != 1)
-        std::cout << "eikonal_type " << type
-                  << ": the device path solves with the block FIM (type 1)\n";
-    eikonal.reset(new Block_FIM());
+    if (type == 0)
+        eikonal.reset(new Classic());
+    else if (type == 1)
+        eikonal.reset(new Block_FIM());
+    else
+        eikonal.reset(new Accurate_FSM());
     eikonal->parameters = parameters;
     eikonal->set_parameters();
     n_data = eikonal->total_shots * eikonal->total_nodes;
@@ -47,6 +51,7 @@ void Least_squares::set_parameters() {
     dm.assign(eikonal->nPoints, 0.f);
     eikonal->prepare_volumes();
     int nt[3];
+    check(st_tomo_set_eikonal(eikonal->context(), type), eikonal->context());
     check(st_tomo_set_grid(eikonal->context(), dx_tomo, dy_tomo, dz_tomo, 0), eikonal->context());
     check(st_tomo_dims(eikonal->context(), &nt[0], &nt[1], &nt[2]), eikonal->context());
     n_model = nt[0] * nt[1] * nt[2];
@@ -67,8 +72,9 @@ void Least_squares::forward_modeling() {
     const std::vector<float> r = eikonal->node_xyz();
     std::vector<float> illum(eikonal->nPoints);
     st_fim_stats st;
+    const Block_FIM *fim = dynamic_cast<const Block_FIM *>(eikonal.get());
     check(st_forward(ctx, s.data(), eikonal->total_shots, r.data(), eikonal->total_nodes,
-                     dcal.data(), illum.data(), &eikonal->opts, &st),
+                     dcal.data(), illum.data(), fim ? &fim->opts : nullptr, &st),
           ctx);
     if (write_files) {
         const int nr = eikonal->total_nodes;
@@ -87,12 +93,17 @@ void Least_squares::import_cal_data() {
 }
 
 bool Least_squares::converged() {
-    double r = 0.0;
-    check(st_residual_norm(eikonal->context(), dobs.data(), dcal.data(), n_data, &r),
-          eikonal->context());
-    residuo.push_back((float)r);
+    // the reference's sequential FP32 sum of powf(d, 2.0f) (which its
+    // optimised build evaluates as d * d); st_residual_norm gives the FP64
+    // value of the same norm
+    float r = 0.0f;
+    for (int i = 0; i < n_data; i++) {
+        const float d = dobs[i] - dcal[i];
+        r += d * d;
+    }
+    residuo.push_back(std::sqrt(r));
     if (iteration >= max_iteration) {
-        std::cout << "\nFinal residuo: " << r << std::endl;
+        std::cout << "\nFinal residuo: " << std::sqrt(r) << std::endl;
         return true;
     }
     iteration += 1;

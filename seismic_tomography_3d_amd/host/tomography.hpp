@@ -8,7 +8,9 @@
 #include <string>
 #include <vector>
 
+#include "accurate_fsm.hpp"
 #include "block_fim.hpp"
+#include "classic.hpp"
 
 namespace st_host {
 
@@ -22,7 +24,7 @@ class Tomography {
     int n_data = 0, n_model = 0;
     std::string obs_data_folder, cal_data_folder, convergency_folder, estimated_model_folder;
     std::vector<float> residuo;
-    std::unique_ptr<Block_FIM> eikonal;
+    std::unique_ptr<DeviceEikonal> eikonal;  // chosen by eikonal_type
     virtual void info_message() = 0;
 
   public:
@@ -41,11 +43,11 @@ class Tomography {
 
 class Least_squares : public Tomography {
   public:
-    void set_parameters() override;     // least_squares.cpp:83-142
+    void set_parameters() override;     // least_squares.cpp:83-142 (solver: :85-104)
     void import_obs_data() override;    // :144-161
     void forward_modeling() override;   // :163-190 (+ ray tracing :192-280)
     void import_cal_data() override;    // :282-299 (picks stay in memory)
-    bool converged() override;          // :301-319
+    bool converged() override;          // :301-319 (the reference's FP32 sum)
     void optimization() override;       // :321-356 (+ tk_reg_matrix, sparse_cgls)
     void model_update() override;       // :498-570
     void export_convergency() override; // :572-579
