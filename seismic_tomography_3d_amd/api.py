@@ -23,7 +23,11 @@ def _f32(a, shape=None):
 
 
 def _ptr(a):
-    return a.ctypes.data_as(_lib.c_float_p) if a is not None else None
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array passed across the C ABI must be C-contiguous")
+    return a.ctypes.data_as(_lib.c_float_p)
 
 
 class Context:

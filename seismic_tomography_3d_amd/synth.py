@@ -48,7 +48,7 @@ def gradient_model(nx, ny, nz, h, v0=1500.0, v1=4000.0, amp=-0.10,
     vel = np.broadcast_to(v[None, None, :], (ny, nx, nz)).astype(np.float64)
     if amp != 0.0:
         vel = vel * (1.0 + amp * gaussian_bump(nx, ny, nz, h, centre, sigma))
-    return vel.astype(np.float32)
+    return np.ascontiguousarray(vel, np.float32)
 
 
 def layered_model(nx, ny, nz, h, n_layers=5, n_anom=3, amp=0.08,
@@ -69,7 +69,7 @@ def layered_model(nx, ny, nz, h, n_layers=5, n_anom=3, amp=0.08,
         sig = rng.uniform(0.08, 0.15) * min(Lx, Ly, max(Lz, h))
         sign = rng.choice([-1.0, 1.0])
         vel = vel * (1.0 + sign * amp * gaussian_bump(nx, ny, nz, h, c, sig))
-    return vel.astype(np.float32)
+    return np.ascontiguousarray(vel, np.float32)
 
 
 def random_layered_model(nx, ny, nz, h, seed):
@@ -84,7 +84,7 @@ def random_layered_model(nx, ny, nz, h, seed):
 def lateral_average(vel):
     """1D start model: the mean over x and y of each depth sample."""
     v = vel.astype(np.float64).mean(axis=(0, 1))
-    return np.broadcast_to(v[None, None, :], vel.shape).astype(np.float32)
+    return np.ascontiguousarray(np.broadcast_to(v[None, None, :], vel.shape), np.float32)
 
 
 def linspace_f32(xi, xf, n):

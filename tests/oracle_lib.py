@@ -24,6 +24,8 @@ _R = None
 def P(a):
     if a is None:
         return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("oracle arguments must be C-contiguous")
     if a.dtype == np.float32:
         return a.ctypes.data_as(fp)
     if a.dtype == np.float64:
