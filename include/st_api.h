@@ -62,6 +62,15 @@ typedef struct {
                            Gauss-Seidel pass over diagonal planes in the octant
                            facing away from the source, then z sweeps; 1 = z
                            sweeps only                                          */
+    int order;          /* block visit ordering (rounds visit, per shot, the
+                           scheduled blocks whose key lies within `band` s of
+                           the shot's smallest key; the others wait; every
+                           mode reaches the same fixed point): 0 (default) =
+                           keys from the network's predicted arrival for warm
+                           starts, no ordering for cold ones; 1 = each
+                           block's tentative arrival, the smallest face value
+                           through which a visit scheduled it (no network
+                           needed); 2 = the larger of the two (warm starts)  */
 } st_fim_opts;
 
 typedef struct {

@@ -30,7 +30,8 @@ class FimOpts(ctypes.Structure):
                 ("judge", ctypes.c_int), ("grid", ctypes.c_int),
                 ("rounds_per_sync", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("band", ctypes.c_float), ("cold_radius", ctypes.c_float),
-                ("warm_values", ctypes.c_int), ("sweep", ctypes.c_int)]
+                ("warm_values", ctypes.c_int), ("sweep", ctypes.c_int),
+                ("order", ctypes.c_int)]
 
 
 class FimStats(ctypes.Structure):

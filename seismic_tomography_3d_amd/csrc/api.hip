@@ -9,7 +9,7 @@
 namespace st {
 void ensure_block_model(st_ctx *ctx, int B);
 void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
-               st_fim_stats *stats, bool warm_loaded, const float *keys);
+               st_fim_stats *stats, bool warm_loaded, const float *keys, bool tentative);
 void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nshot, int B,
                     float delta, float r_cold, bool values, float *dkeys);
 void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res);
@@ -216,7 +216,8 @@ int st_ctx_destroy(st_ctx *ctx) {
                     &ctx->fsm_sid, &ctx->pod_T, &ctx->pod_stamp, &ctx->pod_list,
                     &ctx->pod_newv, &ctx->pod_counts, &ctx->pod_nit, &ctx->pod_temp,
                     &ctx->src, &ctx->rcv,
-                    &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred})
+                    &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred,
+                    &ctx->tkeys})
         b->release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -355,11 +356,13 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
     if (o.delta <= 0.f) o.delta = 0.02f;
     ST_REQUIRE(o.warm == 0 || o.warm == 1, ST_ERR_ARG, "warm must be 0 or 1");
     ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
+    ST_REQUIRE(o.order >= 0 && o.order <= 2, ST_ERR_ARG, "order must be 0, 1 or 2");
     const bool warm = (o.warm == 1) || T_pred;
+    ST_REQUIRE(o.order != 2 || warm, ST_ERR_ARG, "order 2 needs a warm start's predictions");
     set_device(ctx);
     ensure_range(ctx);
     if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
-    if (o.band < 0.f || !warm) o.band = 1e30f;
+    if (o.band < 0.f || (!warm && o.order == 0)) o.band = 1e30f;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (nshot == 0) return;
     check_points(src_xyz, nshot, ctx, "source");
@@ -419,7 +422,8 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
             }
             keys = ctx->keys.as<float>();
         }
-        fim_batch(ctx, dsrc, nb, o, stats, warm, keys);
+        if (o.order == 1) keys = nullptr;  // the solve's own arrivals alone
+        fim_batch(ctx, dsrc, nb, o, stats, warm, keys, o.order != 0);
         std::vector<int> bad;
         // the judge can only reject seeded values: an ordering-only warm start
         // keeps the cold initial state and reaches the cold fixed point
@@ -452,7 +456,7 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
             st_fim_opts oc = o;
             oc.warm = 0;
             oc.band = 1e30f;
-            fim_batch(ctx, gsrc, k, oc, stats, false, nullptr);
+            fim_batch(ctx, gsrc, k, oc, stats, false, nullptr, false);
             if (stats) stats->fallbacks += k;
             for (int q = 0; q < k; q++) {
                 const long long row = b0 + bad[q];
@@ -680,6 +684,8 @@ static void forward_impl(st_ctx *ctx, const float *src_xyz, int nshot, const flo
         ST_REQUIRE(o.warm == 0 || o.warm_values == 0, ST_ERR_ARG,
                    "forward warm starts order visits only (warm_values = 0)");
         ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
+        ST_REQUIRE(o.order >= 0 && o.order <= 2, ST_ERR_ARG, "order must be 0, 1 or 2");
+        ST_REQUIRE(o.order != 2 || o.warm == 1, ST_ERR_ARG, "order 2 needs a warm start");
     } else {
         ST_REQUIRE(o.warm == 0, ST_ERR_ARG, "warm starts apply to the block FIM (eikonal_type 1)");
     }
@@ -701,7 +707,7 @@ static void forward_impl(st_ctx *ctx, const float *src_xyz, int nshot, const flo
     if (type == 1) {
         const bool warm = o.warm == 1;
         if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
-        if (o.band < 0.f || !warm) o.band = 1e30f;
+        if (o.band < 0.f || (!warm && o.order == 0)) o.band = 1e30f;
         const int B = o.block;
         ensure_block_model(ctx, B);
         const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
@@ -724,7 +730,8 @@ static void forward_impl(st_ctx *ctx, const float *src_xyz, int nshot, const flo
                 ST_REQUIRE(ok, ST_ERR_NO_SURROGATE, "warm start requested without a network");
                 keys = ctx->keys.as<float>();
             }
-            fim_batch(ctx, dsrc, nb, o, stats, warm, keys);
+            if (o.order == 1) keys = nullptr;  // the solve's own arrivals alone
+            fim_batch(ctx, dsrc, nb, o, stats, warm, keys, o.order != 0);
             launch_picks_blk(ctx->stream, ctx->T.as<float>(), g.vol, nb, drcv, nrcv,
                              ctx->picks.as<float>() + (long long)b0 * nrcv, ctx->nx, ctx->ny,
                              ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);

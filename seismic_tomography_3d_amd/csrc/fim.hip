@@ -60,6 +60,11 @@ struct FimArgs {
     int nshot;
     float band;
     const float *src;          // the batch's sources, 3 per shot (x, y, z in m)
+    // arrival ordering (nullable; replaces keys when set): per list parity
+    // and (shot, block), the smallest face value (float bits) through which
+    // a visit scheduled the block -- a lower bound on the block's arrival
+    // known from the data itself.  [2][nshot * nblk]
+    unsigned *tk;
 };
 
 constexpr unsigned kInfBits = 0x7F800000u;
@@ -104,14 +109,23 @@ __device__ __forceinline__ float godunov(float a, float b, float c, float hs, fl
 // appends the newly scheduled ones to the next list.
 __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsigned *fl,
                                          unsigned target, unsigned bit_nxt, unsigned *nxt,
-                                         unsigned *nxt_cnt, int lane) {
+                                         unsigned *nxt_cnt, int lane, float key) {
     bool fresh = false;
     if (target != 0xFFFFFFFFu) {
         const unsigned old = atomicOr(&fl[target], bit_nxt);
         fresh = !(old & bit_nxt);
-        if (fresh && A.keys)
+        if (A.tk) {
+            // every scheduler lowers the next list's key (the item itself is
+            // appended once, by the first); with predictions too, the block
+            // counts as due no earlier than its predicted arrival
+            atomicMin(&A.tk[((long long)((A.round + 1) & 1) * A.nshot + shot) * A.nblk + target],
+                      __float_as_uint(key));
+            const float eff = A.keys ? fmaxf(key, A.keys[(long long)shot * A.nblk + target]) : key;
+            atomicMin(&A.kmin[((A.round + 1) % 3) * A.nshot + shot], __float_as_uint(eff));
+        } else if (fresh && A.keys) {
             atomicMin(&A.kmin[((A.round + 1) % 3) * A.nshot + shot],
                       __float_as_uint(A.keys[(long long)shot * A.nblk + target]));
+        }
     }
     const unsigned long long fm = __ballot(fresh);
     if (fm) {
@@ -124,6 +138,18 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
         }
     }
 }
+
+// Minimum over the lanes whose index differs from this lane's only in the
+// bits of mask (a butterfly over those bits).
+template <int MASK>
+__device__ __forceinline__ float lane_min(float v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+        if (MASK & o) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+constexpr float kKeyInf = __builtin_inff();  // "no key" for the arrival ordering
 
 // Round kernels run kWaves independent waves per workgroup (each with its
 // own LDS tile and item stream): workgroup slots per CU, not registers, cap
@@ -178,7 +204,7 @@ __device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, unsi
         }
         const unsigned item = in ? cur[seg * A.cap + (i - before)] : 0u;
         bool run = in;
-        if (A.keys) {
+        if (A.keys || A.tk) {
             bool fresh = false;
             unsigned shot = 0;
             if (in) {
@@ -186,15 +212,30 @@ __device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, unsi
                 const unsigned b = item & kBlkMask;
                 const float tau =
                     __uint_as_float(A.kmin[(A.round % 3) * A.nshot + shot]) + A.band;
-                const float key = A.keys[(long long)shot * A.nblk + b];
+                float key, kt = 0.0f;
+                if (A.tk) {
+                    // this list's key is consumed here (visited or carried
+                    // over); only the next list's keys are written this round
+                    unsigned *kc = A.tk + ((long long)(A.round & 1) * A.nshot + shot) * A.nblk + b;
+                    kt = __uint_as_float(*kc);
+                    *kc = kInfBits;
+                    key = A.keys ? fmaxf(kt, A.keys[(long long)shot * A.nblk + b]) : kt;
+                } else {
+                    key = A.keys[(long long)shot * A.nblk + b];
+                }
                 if (key > tau) {  // not its time yet: carried to the next round
                     run = false;
                     unsigned *fl = A.flags + (long long)shot * A.nblk;
                     atomicAnd(&fl[b], ~bit_cur);
                     fresh = !(atomicOr(&fl[b], bit_nxt) & bit_nxt);
-                    if (fresh)
+                    if (A.tk) {
+                        atomicMin(&A.tk[((long long)((A.round + 1) & 1) * A.nshot + shot) * A.nblk + b],
+                                  __float_as_uint(kt));
+                        atomicMin(&A.kmin[((A.round + 1) % 3) * A.nshot + shot], __float_as_uint(key));
+                    } else if (fresh) {
                         atomicMin(&A.kmin[((A.round + 1) % 3) * A.nshot + shot],
                                   __float_as_uint(key));
+                    }
                 }
             }
             const unsigned long long fm = __ballot(fresh);
@@ -229,7 +270,7 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round4(FimArgs A) {
     if (gw == 0) {
         if (lane < kSeg) A.cnt[(((A.round + 2) % 3) * kSeg + lane) * kCntStride] = 0;
         if (lane == 0 && count) atomicAdd(&A.cnt[kRoundsIdx], 1u);
-        if (A.keys)
+        if (A.keys || A.tk)
             for (int j = lane; j < A.nshot; j += 64) A.kmin[((A.round + 2) % 3) * A.nshot + j] = kInfBits;
     }
     if (count == 0) return;
@@ -323,7 +364,21 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round4(FimArgs A) {
         case 6: if (zp && bz < A.nbz - 1) target = b + nbxy; break;
         default: break;
         }
-        schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane);
+        float key = kKeyInf;
+        if (A.tk) {
+            // per face, the smallest new value among the cells that activate
+            // the neighbour; for a self-reschedule, among the changed cells
+            auto face = [&](bool c) { return lane_min<63>(c ? v : kKeyInf); };
+            const float k0 = face(ch), k1 = face(ch && tx == 0 && v < tile[tc - 1]),
+                        k2 = face(ch && tx == 3 && v < tile[tc + 1]),
+                        k3 = face(ch && ty == 0 && v < tile[tc - 6]),
+                        k4 = face(ch && ty == 3 && v < tile[tc + 6]),
+                        k5 = face(ch && tz == 0 && v < tile[tc - 36]),
+                        k6 = face(ch && tz == 3 && v < tile[tc + 36]);
+            key = lane == 0 ? k0 : lane == 1 ? k1 : lane == 2 ? k2 : lane == 3 ? k3
+                : lane == 4 ? k4 : lane == 5 ? k5 : k6;
+        }
+        schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane, key);
         my_visits += 1;
         my_iters += it;
         wave_sync();
@@ -365,7 +420,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
     if (gw == 0) {
         if (lane < kSeg) A.cnt[(((A.round + 2) % 3) * kSeg + lane) * kCntStride] = 0;
         if (lane == 0 && count) atomicAdd(&A.cnt[kRoundsIdx], 1u);
-        if (A.keys)
+        if (A.keys || A.tk)
             for (int j = lane; j < A.nshot; j += 64) A.kmin[((A.round + 2) % 3) * A.nshot + j] = kInfBits;
     }
     if (count == 0) return;
@@ -520,16 +575,27 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         // face activation as in the 4^3 kernel: a changed face cell must be
         // below the neighbour's adjacent cell (tile halo / z halo registers)
         bool anydec = false, fxm = false, fxp = false, fym = false, fyp = false;
+        // arrival-ordering keys: smallest activating value on the x faces
+        // (tx = 0 / 7 lanes), the y faces (ty = 0 / 7) and among the changed
+        // cells of the column
+        float kx = kKeyInf, ky = kKeyInf, kc = kKeyInf;
         #pragma unroll
         for (int k = 0; k < 8; k++) {
             const float dl = v0[k] - v[k];
             anydec |= dl > 0.0f;
             const bool c = dl > A.eps;
             const int ti = tcol + 100 * (k + 1);
-            fxm |= c && tx == 0 && v[k] < tile[ti - 1];
-            fxp |= c && tx == 7 && v[k] < tile[ti + 1];
-            fym |= c && ty == 0 && v[k] < tile[ti - 10];
-            fyp |= c && ty == 7 && v[k] < tile[ti + 10];
+            const bool ax = c && ((tx == 0 && v[k] < tile[ti - 1]) || (tx == 7 && v[k] < tile[ti + 1]));
+            const bool ay = c && ((ty == 0 && v[k] < tile[ti - 10]) || (ty == 7 && v[k] < tile[ti + 10]));
+            fxm |= ax && tx == 0;
+            fxp |= ax && tx == 7;
+            fym |= ay && ty == 0;
+            fyp |= ay && ty == 7;
+            if (A.tk) {
+                kx = ax ? fminf(kx, v[k]) : kx;
+                ky = ay ? fminf(ky, v[k]) : ky;
+                kc = c ? fminf(kc, v[k]) : kc;
+            }
         }
         const bool fzm = (v0[0] - v[0] > A.eps) && v[0] < vz[0];
         const bool fzp = (v0[7] - v[7] > A.eps) && v[7] < vz[9];
@@ -551,7 +617,19 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         case 6: if (zp && bz < A.nbz - 1) target = b + nbxy; break;
         default: break;
         }
-        schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane);
+        float key = kKeyInf;
+        if (A.tk) {
+            // x faces: min over ty (lanes tx + 8 ty), y faces: min over tx
+            const float mx = lane_min<56>(kx), my = lane_min<7>(ky);
+            const float mzm = lane_min<63>(fzm ? v[0] : kKeyInf);
+            const float mzp = lane_min<63>(fzp ? v[7] : kKeyInf);
+            const float mc = lane_min<63>(kc);
+            const float k1 = __shfl(mx, 0), k2 = __shfl(mx, 7), k3 = __shfl(my, 0),
+                        k4 = __shfl(my, 56);
+            key = lane == 0 ? mc : lane == 1 ? k1 : lane == 2 ? k2 : lane == 3 ? k3
+                : lane == 4 ? k4 : lane == 5 ? mzm : mzp;
+        }
+        schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane, key);
         my_visits += 1;
         my_iters += it;
         wave_sync();
@@ -591,12 +669,18 @@ __global__ void fill_inf(float4 *__restrict__ T, long long n4) {
     if (i < n4) T[i] = make_float4(kInf, kInf, kInf, kInf);
 }
 
+__global__ void fill_u32(unsigned *__restrict__ p, long long n, unsigned v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // apply_source_time (block_FIM.cpp:295-369): the source's grid node gets
 // S*distance, its block is the only active one.
 __global__ void seed_sources(const float *__restrict__ src, int nshot, const float *__restrict__ S,
                              float *T, unsigned *flags, unsigned *list0, unsigned *cnt,
                              float dx, float dy, float dz, int B, int nbx, int nby,
-                             long long vol, long long nblk, const float *keys, unsigned *kmin) {
+                             long long vol, long long nblk, const float *keys, unsigned *kmin,
+                             unsigned *tk) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) {  // the sources form segment 0 of the first list
         for (int k = 0; k < 3 * kSeg; k++) cnt[k * kCntStride] = 0;
@@ -612,7 +696,12 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
     const unsigned b = (unsigned)(a / (B * B * B));
     flags[(long long)j * nblk + b] = 1u;
     list0[j] = ((unsigned)j << kShotShift) | b;
-    if (keys) {
+    if (tk) {  // the source block opens the first list with key 0
+        tk[(long long)j * nblk + b] = 0u;
+        kmin[j] = keys ? __float_as_uint(keys[(long long)j * nblk + b]) : 0u;
+        kmin[nshot + j] = kInfBits;
+        kmin[2 * nshot + j] = kInfBits;
+    } else if (keys) {
         kmin[j] = __float_as_uint(keys[(long long)j * nblk + b]);
         kmin[nshot + j] = kInfBits;
         kmin[2 * nshot + j] = kInfBits;
@@ -821,53 +910,97 @@ namespace st {
 
 static int blk_index(int B) { return B == 8 ? 1 : 0; }
 
+// The model's slowness in the block layout of edge B, into S (vol floats).
+void pack_block_model(st_ctx *ctx, int B, float *S) {
+    const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+    const int tpb = 256;
+    hipLaunchKernelGGL(pack_model, dim3((unsigned)((g.vol + tpb - 1) / tpb)), dim3(tpb), 0,
+                       ctx->stream, ctx->slow_zxy.as<float>(), S, ctx->nx, ctx->ny, ctx->nz, B,
+                       g.nbx, g.nby, g.vol);
+    ST_HIP(hipGetLastError());
+}
+
 void ensure_block_model(st_ctx *ctx, int B) {
     const int bi = blk_index(B);
     if (ctx->blk_valid[bi]) return;
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     ctx->slow_blk[bi].reserve(sizeof(float) * g.vol);
-    const int tpb = 256;
-    hipLaunchKernelGGL(pack_model, dim3((unsigned)((g.vol + tpb - 1) / tpb)), dim3(tpb), 0,
-                       ctx->stream, ctx->slow_zxy.as<float>(), ctx->slow_blk[bi].as<float>(),
-                       ctx->nx, ctx->ny, ctx->nz, B, g.nbx, g.nby, g.vol);
-    ST_HIP(hipGetLastError());
+    pack_block_model(ctx, B, ctx->slow_blk[bi].as<float>());
     ctx->blk_valid[bi] = true;
 }
 
 const float *block_model(st_ctx *ctx, int B) { return ctx->slow_blk[blk_index(B)].as<float>(); }
 
-// Device-resident FIM over one batch of shots already uploaded to dsrc.
-// Leaves the travel times in ctx->T (shot-major block layout).
+void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
+                  const st_fim_opts &o, st_fim_stats *stats, bool warm_loaded, const float *keys,
+                  bool tentative);
+
+int fim_waves(const st_fim_opts &o) {
+    // o.grid counts waves; a launch has o.grid / kWaves workgroups
+    const int grid = o.grid > 0 ? o.grid : 256 * 32;  // 32 waves per CU
+    return std::max(1, (grid + kWaves - 1) / kWaves) * kWaves;
+}
+
+// Device-resident FIM over one batch of shots already uploaded to dsrc, on
+// the context's buffers.  Leaves the travel times in ctx->T (shot-major
+// block layout).
 void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
-               st_fim_stats *stats, bool warm_loaded, const float *keys) {
+               st_fim_stats *stats, bool warm_loaded, const float *keys, bool tentative) {
+    const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, o.block);
+    ctx->list[0].reserve(fim_list_bytes(g.nblk, nshot));
+    ctx->list[1].reserve(fim_list_bytes(g.nblk, nshot));
+    if (keys || tentative) ctx->kmin.reserve(sizeof(unsigned) * 3 * std::max(nshot, 1));
+    if (tentative) ctx->tkeys.reserve(sizeof(unsigned) * 2 * nshot * g.nblk);
+    if (stats) ctx->wstats.reserve(sizeof(unsigned long long) * 3 * fim_waves(o));
+    FimBufs bf;
+    bf.S = block_model(ctx, o.block);
+    bf.T = ctx->T.as<float>();
+    bf.flags = ctx->flags.as<unsigned>();
+    bf.list0 = ctx->list[0].as<unsigned>();
+    bf.list1 = ctx->list[1].as<unsigned>();
+    bf.kmin = ctx->kmin.as<unsigned>();
+    bf.tk = ctx->tkeys.as<unsigned>();
+    bf.wstats = ctx->wstats.as<unsigned long long>();
+    bf.graphs = true;
+    fim_batch_on(ctx, bf, dsrc, nshot, o, stats, warm_loaded, keys, tentative);
+}
+
+// The same on caller-provided device buffers (sized as fim_batch reserves
+// them); graph replay only when bf.graphs.
+void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
+                  const st_fim_opts &o, st_fim_stats *stats, bool warm_loaded, const float *keys,
+                  bool tentative) {
     const int B = o.block;
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     unsigned *cnt = ctx->counters.as<unsigned>();
     unsigned long long *dstats = reinterpret_cast<unsigned long long *>(cnt + kStatsIdx);
-    ctx->list[0].reserve(fim_list_bytes(g.nblk, nshot));
-    ctx->list[1].reserve(fim_list_bytes(g.nblk, nshot));
     const int tpb = 256;
 
-    if (keys) ctx->kmin.reserve(sizeof(unsigned) * 3 * std::max(nshot, 1));
+    unsigned *tk = nullptr;
+    if (tentative) {
+        const long long n = 2LL * nshot * g.nblk;
+        tk = bf.tk;
+        hipLaunchKernelGGL(fill_u32, dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
+                           ctx->stream, tk, n, kInfBits);
+    }
     if (!warm_loaded) {
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + tpb - 1) / tpb)), dim3(tpb), 0,
-                           ctx->stream, ctx->T.as<float4>(), n4);
+                           ctx->stream, reinterpret_cast<float4 *>(bf.T), n4);
     }
-    ST_HIP(hipMemsetAsync(ctx->flags.ptr, 0, sizeof(unsigned) * g.nblk * nshot, ctx->stream));
+    ST_HIP(hipMemsetAsync(bf.flags, 0, sizeof(unsigned) * g.nblk * nshot, ctx->stream));
     hipLaunchKernelGGL(seed_sources, dim3((nshot + 63) / 64), dim3(64), 0, ctx->stream, dsrc,
-                       nshot, block_model(ctx, B), ctx->T.as<float>(), ctx->flags.as<unsigned>(),
-                       ctx->list[0].as<unsigned>(), cnt, ctx->dx, ctx->dy, ctx->dz, B, g.nbx,
-                       g.nby, g.vol, g.nblk, keys, ctx->kmin.as<unsigned>());
+                       nshot, bf.S, bf.T, bf.flags, bf.list0, cnt, ctx->dx, ctx->dy, ctx->dz, B,
+                       g.nbx, g.nby, g.vol, g.nblk, keys, bf.kmin, tk);
     ST_HIP(hipGetLastError());
 
     FimArgs A;
-    A.S = block_model(ctx, B);
-    A.T = ctx->T.as<float>();
-    A.flags = ctx->flags.as<unsigned>();
+    A.S = bf.S;
+    A.T = bf.T;
+    A.flags = bf.flags;
     A.cap = g.nblk * (long long)nshot;
-    A.list0 = ctx->list[0].as<unsigned>();
-    A.list1 = ctx->list[1].as<unsigned>();
+    A.list0 = bf.list0;
+    A.list1 = bf.list1;
     A.cnt = cnt;
     A.stats = nullptr;
     A.nbx = g.nbx; A.nby = g.nby; A.nbz = g.nbz;
@@ -877,21 +1010,17 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     A.eps = o.eps;
     A.max_inner = o.max_inner;
     A.keys = keys;
-    A.kmin = ctx->kmin.as<unsigned>();
+    A.kmin = bf.kmin;
     A.nshot = nshot;
     A.band = o.band;
     A.src = dsrc;
+    A.tk = tk;
 
-    // o.grid counts waves; a launch has o.grid / kWaves workgroups
-    int grid = o.grid;
-    if (grid <= 0) grid = 256 * 32;  // 32 waves per CU
-    grid = std::max(1, (grid + kWaves - 1) / kWaves);
-    const int nwaves = grid * kWaves;
+    const int nwaves = fim_waves(o);
+    const int grid = nwaves / kWaves;
     if (stats) {
-        ctx->wstats.reserve(sizeof(unsigned long long) * 3 * nwaves);
-        ST_HIP(hipMemsetAsync(ctx->wstats.ptr, 0, sizeof(unsigned long long) * 3 * nwaves,
-                              ctx->stream));
-        A.stats = ctx->wstats.as<unsigned long long>();
+        ST_HIP(hipMemsetAsync(bf.wstats, 0, sizeof(unsigned long long) * 3 * nwaves, ctx->stream));
+        A.stats = bf.wstats;
     }
     const int K = o.rounds_per_sync > 0 ? o.rounds_per_sync : 6;
     unsigned *hcnt = static_cast<unsigned *>(ctx->host_pinned);
@@ -922,7 +1051,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     // round kernels clear their own counters, so the graph holds kernel nodes
     // only.
     hipGraphExec_t exec = nullptr;
-    if (o.use_graph) {
+    if (o.use_graph && bf.graphs) {
         long long key = 1469598103934665603LL;
         auto mix = [&](long long v) { key = (key ^ v) * 1099511628211LL; };
         mix((long long)(uintptr_t)A.T); mix((long long)(uintptr_t)A.S); mix((long long)(uintptr_t)A.keys);
@@ -936,7 +1065,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
         mix(A.vol); mix(A.cap); mix(A.nshot);
         mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
         mix((long long)__builtin_bit_cast(unsigned, A.h));
-        mix((long long)(uintptr_t)A.src); mix(hyp ? 1 : 0);
+        mix((long long)(uintptr_t)A.src); mix(hyp ? 1 : 0); mix((long long)(uintptr_t)A.tk);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) ST_HIP(hipGraphExecDestroy(ctx->graph_exec));
             if (ctx->graph) ST_HIP(hipGraphDestroy(ctx->graph));
@@ -978,8 +1107,8 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     }
     ST_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     if (stats) {
-        hipLaunchKernelGGL(sum_wave_stats, dim3(1), dim3(256), 0, ctx->stream,
-                           ctx->wstats.as<unsigned long long>(), nwaves, dstats);
+        hipLaunchKernelGGL(sum_wave_stats, dim3(1), dim3(256), 0, ctx->stream, bf.wstats, nwaves,
+                           dstats);
         ST_HIP(hipGetLastError());
         ST_HIP(hipMemcpyAsync(hcnt + 4, cnt + kRoundsIdx, sizeof(unsigned), hipMemcpyDeviceToHost,
                               ctx->stream));

@@ -117,6 +117,21 @@ void model_changed(st_ctx *ctx);
 void ensure_range(st_ctx *ctx);
 }  // namespace st
 
+namespace st {
+// Device buffers of one batched FIM solve (sizes as fim_batch reserves them
+// for nshot shots: T nshot*vol, flags nshot*nblk, each list
+// fim_list_bytes(nblk, nshot), kmin 3*nshot, tk 2*nshot*nblk, wstats
+// 3*fim_waves(opts) counters).
+struct FimBufs {
+    const float *S = nullptr;      // block-layout slowness
+    float *T = nullptr;
+    unsigned *flags = nullptr, *list0 = nullptr, *list1 = nullptr;
+    unsigned *kmin = nullptr, *tk = nullptr;
+    unsigned long long *wstats = nullptr;
+    bool graphs = false;           // the context's round-graph cache may be used
+};
+}  // namespace st
+
 // The context.
 struct st_ctx {
     int device = 0;
@@ -145,6 +160,7 @@ struct st_ctx {
     bool fsm_valid = false;            // fsm_S matches the model
     st::DevBuf src, rcv, picks, scratch;
     st::DevBuf keys, kmin, judge, pred;   // warm-start state
+    st::DevBuf tkeys;                  // arrival-ordering keys (2 x batch x nblk)
     void *host_pinned = nullptr;       // small pinned staging area
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 

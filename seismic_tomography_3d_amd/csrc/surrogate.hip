@@ -26,7 +26,7 @@ namespace st {
 
 void ensure_block_model(st_ctx *ctx, int B);
 void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
-               st_fim_stats *stats, bool warm_loaded, const float *keys);
+               st_fim_stats *stats, bool warm_loaded, const float *keys, bool tentative);
 void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dmin,
                       const float *dsrc, int nshot, int B, float delta, float r_cold, bool values,
                       float *dkeys);
@@ -721,7 +721,7 @@ int st_surrogate_make_data(st_ctx *ctx, int n_models, int spm, float zmax_src, u
             }
             ST_HIP(hipMemcpyAsync(ctx->src.ptr, hs.data(), sizeof(float) * 3 * spm,
                                   hipMemcpyHostToDevice, ctx->stream));
-            fim_batch(ctx, ctx->src.as<float>(), spm, o, nullptr, false, nullptr);
+            fim_batch(ctx, ctx->src.as<float>(), spm, o, nullptr, false, nullptr, false);
             const long long s0 = (long long)mdl * spm;
             hipLaunchKernelGGL(prep_inputs, dim3(nblocks(V * spm)), dim3(256), 0, ctx->stream,
                                ctx->slow_zxy.as<float>(), ctx->src.as<float>(), spm, cg,
