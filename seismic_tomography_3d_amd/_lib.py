@@ -57,6 +57,9 @@ SIGNATURES = [
     ("st_fim_solve", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
                                     ctypes.c_int, c_float_p, c_float_p,
                                     ctypes.POINTER(FimOpts), ctypes.POINTER(FimStats)]),
+    ("st_fim_solve_noalloc", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
+                                            ctypes.c_int, c_float_p, ctypes.POINTER(FimOpts),
+                                            ctypes.POINTER(FimStats)]),
     ("st_fim_solve_warm", ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int, c_float_p,
                                          ctypes.c_int, c_float_p, c_float_p, c_float_p,
                                          ctypes.POINTER(FimOpts), ctypes.POINTER(FimStats)]),

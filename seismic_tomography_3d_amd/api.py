@@ -104,6 +104,20 @@ class Context:
                                           ctypes.byref(o), ctypes.byref(st)))
         return picks, T, st.as_dict()
 
+    def fim_solve_noalloc(self, src, rcv, **opts):
+        """Cold-start solve whose device buffers all come from the library's
+        static scratch array (no device allocation); returns (picks, stats)."""
+        src = _f32(src, (-1, 3))
+        rcv = _f32(rcv, (-1, 3)) if rcv is not None else np.zeros((0, 3), np.float32)
+        ns, nr = src.shape[0], rcv.shape[0]
+        picks = np.empty((ns, nr), np.float32)
+        o = self.options(**opts)
+        st = FimStats()
+        self._check(self.lib.st_fim_solve_noalloc(self.h, _ptr(src), ns, _ptr(rcv) if nr else None,
+                                                  nr, _ptr(picks) if nr else None,
+                                                  ctypes.byref(o), ctypes.byref(st)))
+        return picks, st.as_dict()
+
     def fim_solve_warm(self, src, rcv, T_pred, return_T=False, **opts):
         """Warm-started solve from caller predictions T_pred[nshot, ny, nx, nz]."""
         src = _f32(src, (-1, 3))

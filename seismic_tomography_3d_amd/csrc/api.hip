@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "st_common.hpp"
 
@@ -10,6 +11,11 @@ namespace st {
 void ensure_block_model(st_ctx *ctx, int B);
 void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
                st_fim_stats *stats, bool warm_loaded, const float *keys, bool tentative);
+void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
+                  const st_fim_opts &o, st_fim_stats *stats, bool warm_loaded, const float *keys,
+                  bool tentative);
+int fim_waves(const st_fim_opts &o);
+void pack_block_model(st_ctx *ctx, int B, float *S);
 void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nshot, int B,
                     float delta, float r_cold, bool values, float *dkeys);
 void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res);
@@ -134,11 +140,10 @@ void model_changed(st_ctx *ctx) {
     ctx->range_valid = false;
 }
 
-void ensure_range(st_ctx *ctx) {
+// smin/smax of the device slowness through two words of device memory d.
+static void ensure_range_on(st_ctx *ctx, unsigned *d) {
     if (ctx->range_valid) return;
     const long long n = (long long)ctx->nx * ctx->ny * ctx->nz;
-    ctx->range.reserve(64);
-    unsigned *d = ctx->range.as<unsigned>();
     hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, ctx->stream, d);
     const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 2048);
     hipLaunchKernelGGL(slowness_range, dim3(grid), dim3(256), 0, ctx->stream,
@@ -153,6 +158,53 @@ void ensure_range(st_ctx *ctx) {
     ctx->smax = __builtin_bit_cast(float, h[1]);
     ctx->range_valid = true;
 }
+
+void ensure_range(st_ctx *ctx) {
+    if (ctx->range_valid) return;
+    ctx->range.reserve(64);
+    ensure_range_on(ctx, ctx->range.as<unsigned>());
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation FIM solve.  Every device buffer of st_fim_solve_noalloc --
+// block-layout slowness, travel times, schedule flags, active lists,
+// ordering keys, per-wave statistics, sources, receivers, picks and the
+// slowness-range words -- is carved out of one file-scope device array.
+// Each GPU the library runs on holds its own copy (kNoAllocBytes) from module
+// load to process end.  Its address on a device is taken once, at context
+// creation with that device current; host code passes that address to
+// kernels and copies, never the array's name (a host-side stand-in).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 256;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// This device's scratch address (the device must be current).
+static void *noalloc_scratch(int device) {
+    ST_REQUIRE(device >= 0 && device < kMaxDevices, ST_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        ST_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ST_REQUIRE(p != nullptr, ST_ERR_HIP, "no device address for the scratch array");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// Bump allocator over the scratch bytes (256-byte aligned pieces).
+struct ScratchCarve {
+    unsigned char *base;
+    size_t used = 0;
+    template <class T> T *take(size_t count) {
+        const size_t off = (used + 255) & ~size_t(255);
+        used = off + sizeof(T) * count;
+        return reinterpret_cast<T *>(base + off);
+    }
+};
 
 }  // namespace st
 
@@ -193,6 +245,7 @@ int st_ctx_create(int device, st_ctx **out) {
         for (auto &e : ctx->ev) ST_HIP(hipEventCreate(&e));
         ctx->counters.reserve(8192);
         ST_HIP(hipMemset(ctx->counters.ptr, 0, 8192));
+        ctx->noalloc = noalloc_scratch(device);
     });
     if (rc != ST_OK) {
         st_ctx_destroy(ctx);
@@ -491,6 +544,102 @@ int st_fim_solve(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_
     if (!ctx) return ST_ERR_ARG;
     return guarded(ctx, [&] {
         solve_impl(ctx, src_xyz, nshot, rcv_xyz, nrcv, picks, T_out, nullptr, opts, stats);
+    });
+}
+
+int st_fim_solve_noalloc(st_ctx *ctx, const float *src_xyz, int nshot, const float *rcv_xyz,
+                         int nrcv, float *picks, const st_fim_opts *opts, st_fim_stats *stats) {
+    if (!ctx) return ST_ERR_ARG;
+    return guarded(ctx, [&] {
+        const auto t_start = std::chrono::steady_clock::now();
+        ST_REQUIRE(ctx->has_model, ST_ERR_NO_MODEL, "st_set_model first");
+        ST_REQUIRE(nshot >= 0 && nrcv >= 0, ST_ERR_ARG, "negative count");
+        ST_REQUIRE(nshot == 0 || src_xyz, ST_ERR_ARG, "null sources");
+        ST_REQUIRE(nrcv == 0 || (rcv_xyz && picks), ST_ERR_ARG, "null receivers or picks");
+        ST_REQUIRE(ctx->noalloc, ST_ERR_STATE, "context without a scratch address");
+        st_fim_opts o;
+        if (opts) o = *opts; else st_fim_default_opts(&o);
+        ST_REQUIRE(o.block == 4 || o.block == 8, ST_ERR_ARG, "block must be 4 or 8");
+        ST_REQUIRE(o.warm == 0, ST_ERR_ARG, "the no-allocation solve starts cold (warm = 0)");
+        ST_REQUIRE(o.sweep == 0 || o.sweep == 1, ST_ERR_ARG, "sweep must be 0 or 1");
+        ST_REQUIRE(o.order == 0 || o.order == 1, ST_ERR_ARG, "order must be 0 or 1");
+        if (o.max_inner <= 0) o.max_inner = (o.block == 8) ? 24 : 16;
+        if (o.eps < 0.f) o.eps = kFimEps;
+        o.use_graph = 0;  // graph instantiation may allocate
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        if (nshot == 0) return;
+        check_points(src_xyz, nshot, ctx, "source");
+        check_points(rcv_xyz, nrcv, ctx, "receiver");
+        set_device(ctx);
+
+        const int B = o.block;
+        const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
+        ST_REQUIRE(g.nblk < (1LL << 22), ST_ERR_ARG, "model has too many blocks");
+        const bool tentative = o.order == 1;
+        const int nwaves = fim_waves(o);
+        // the layout for b shots per pass (each piece 256-byte aligned)
+        auto layout = [&](int b, FimBufs *bf, float **S, float **src, float **rcv, float **pk,
+                          unsigned **range) {
+            ScratchCarve c{static_cast<unsigned char *>(ctx->noalloc)};
+            float *s_ = c.take<float>(g.vol);
+            unsigned *r_ = c.take<unsigned>(2);
+            float *rc_ = c.take<float>(3 * (size_t)std::max(nrcv, 1));
+            float *sr_ = c.take<float>(3 * (size_t)b);
+            float *pk_ = c.take<float>((size_t)b * std::max(nrcv, 1));
+            FimBufs f;
+            f.S = s_;
+            f.T = c.take<float>((size_t)b * g.vol);
+            f.flags = c.take<unsigned>((size_t)b * g.nblk);
+            f.list0 = c.take<unsigned>(fim_list_bytes(g.nblk, b) / sizeof(unsigned));
+            f.list1 = c.take<unsigned>(fim_list_bytes(g.nblk, b) / sizeof(unsigned));
+            f.kmin = tentative ? c.take<unsigned>(3 * (size_t)b) : nullptr;
+            f.tk = tentative ? c.take<unsigned>(2 * (size_t)b * g.nblk) : nullptr;
+            f.wstats = stats ? c.take<unsigned long long>(3 * (size_t)nwaves) : nullptr;
+            f.graphs = false;
+            if (bf) {
+                *bf = f; *S = s_; *src = sr_; *rcv = rc_; *pk = pk_; *range = r_;
+            }
+            return c.used;
+        };
+        int batch = std::max(1, std::min({o.batch, nshot, 1023}));
+        while (batch > 1 && layout(batch, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) >
+                                kNoAllocBytes)
+            batch = std::max(1, batch / 2);
+        const size_t need = layout(batch, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        ST_REQUIRE(need <= kNoAllocBytes, ST_ERR_ARG,
+                   "model too large for the no-allocation solve: one shot needs " +
+                       std::to_string(need >> 20) + " MiB of its " +
+                       std::to_string(kNoAllocBytes >> 20) + " MiB scratch");
+        FimBufs bf;
+        float *S, *dsrc, *drcv, *dpk;
+        unsigned *drange;
+        layout(batch, &bf, &S, &dsrc, &drcv, &dpk, &drange);
+
+        ensure_range_on(ctx, drange);
+        if (o.band == 0.f) o.band = o.block * ctx->dx * ctx->smax;  // one slow block crossing
+        if (o.band < 0.f || !tentative) o.band = 1e30f;
+        pack_block_model(ctx, B, S);
+        if (nrcv)
+            ST_HIP(hipMemcpyAsync(drcv, rcv_xyz, sizeof(float) * 3 * nrcv, hipMemcpyHostToDevice,
+                                  ctx->stream));
+        for (int b0 = 0; b0 < nshot; b0 += batch) {
+            const int nb = std::min(batch, nshot - b0);
+            ST_HIP(hipMemcpyAsync(dsrc, src_xyz + 3LL * b0, sizeof(float) * 3 * nb,
+                                  hipMemcpyHostToDevice, ctx->stream));
+            fim_batch_on(ctx, bf, dsrc, nb, o, stats, false, nullptr, tentative);
+            if (nrcv) {
+                launch_picks_blk(ctx->stream, bf.T, g.vol, nb, drcv, nrcv, dpk, ctx->nx, ctx->ny,
+                                 ctx->nz, ctx->dx, ctx->dy, ctx->dz, B, g.nbx, g.nby);
+                ST_HIP(hipMemcpyAsync(picks + (long long)b0 * nrcv, dpk,
+                                      sizeof(float) * (long long)nb * nrcv, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+            }
+            ST_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (stats)
+            stats->total_ms = std::chrono::duration<double, std::milli>(
+                                  std::chrono::steady_clock::now() - t_start)
+                                  .count();
     });
 }
 

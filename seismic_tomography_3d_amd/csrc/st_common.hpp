@@ -162,6 +162,7 @@ struct st_ctx {
     st::DevBuf keys, kmin, judge, pred;   // warm-start state
     st::DevBuf tkeys;                  // arrival-ordering keys (2 x batch x nblk)
     void *host_pinned = nullptr;       // small pinned staging area
+    void *noalloc = nullptr;           // this device's no-allocation scratch array
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     // graph cache: captured rounds for one (geometry, batch) shape
