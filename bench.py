@@ -9,9 +9,11 @@ path over the whole shot set: block-FIM solve of every shot plus receiver
 picks, with the model already resident in HBM.
 
 Legs reported in the one JSON line:
-  value           shot-solves/s of the headline method (batched persistent
-                  FIM, plus the learned warm start when a trained network is
-                  present: --warm auto)
+  value           shot-solves/s of the headline method: the batched FIM with
+                  the fastest measured visit ordering among none, the solve's
+                  own tentative arrivals and the learned warm start's
+                  predicted arrivals (--order auto; each timed the same way,
+                  all reported under "ordering")
   plain_fim       the same solver one shot at a time, cold start
   roofline        HBM roofline of the dominant kernel (the FIM round kernel):
                   algorithmic bytes per block visit x visits / kernel time
@@ -491,16 +493,19 @@ def main():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--max-inner", type=int, default=0)
     ap.add_argument("--plain-shots", type=int, default=16)
-    ap.add_argument("--warm", default="auto", choices=["auto", "on", "off"])
+    ap.add_argument("--warm", default="auto", choices=["auto", "on", "off"],
+                    help="load the warm-start network (auto: when trained weights exist)")
+    ap.add_argument("--order", default="auto", choices=["auto", "none", "arrival", "network"],
+                    help="visit ordering of the headline solve (auto: the fastest measured)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--check-shots", type=int, default=256,
+    ap.add_argument("--check-shots", type=int, default=32,
                     help="shots checked against the FP64 oracle after the timed steps")
     ap.add_argument("--band", type=float, default=0.0,
-                    help="warm-start ordering band in s (0 = tuned)")
+                    help="ordering band in s for the arrival and network orderings (0 = tuned)")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-shot-at-a-time leg")
     ap.add_argument("--no-cold-leg", action="store_true",
-                    help="skip the batched solve without the network")
+                    help="skip the batched solve without ordering (unless --order none)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Accurate_FSM and tomography-iteration legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -542,20 +547,53 @@ def main():
 
     params = tune.best_params(ctx, src, rcv, block=a.block, grid=a.grid,
                               max_inner=a.max_inner, log=log)
-    warm = 0
+    have_net = False
     if a.warm in ("auto", "on"):
-        have = tune.load_surrogate(ctx, case, log=log)
-        if a.warm == "on" and not have:
+        have_net = tune.load_surrogate(ctx, case, log=log)
+        if a.warm == "on" and not have_net:
             raise SystemExit("--warm on but no trained network is available")
-        warm = 1 if have else 0
+        if a.warm == "on" and a.order == "auto":
+            a.order = "network"
     cold_opts = dict(block=params["block"], grid=params["grid"],
                      max_inner=params["max_inner"], batch=a.batch)
-    opts = dict(cold_opts)
-    if warm:
+
+    # visit ordering: none, the solve's own tentative arrivals, or the
+    # warm-start network's predicted arrivals; each candidate's band is tuned
+    # over whole calls, then every candidate is timed the same way on all
+    # shots and the fastest is the headline configuration
+    cands = {}
+    if a.order == "none" or (a.order == "auto" and not a.no_cold_leg):
+        cands["none"] = dict(cold_opts)
+    if a.order in ("auto", "arrival"):
+        op = (dict(order=1, band=a.band) if a.band > 0
+              else tune.best_order_params(ctx, src, rcv, cold_opts, log=log)[0])
+        cands["arrival"] = dict(cold_opts, **op)
+    if have_net and a.order in ("auto", "network"):
         wp = (dict(band=a.band, max_inner=params["max_inner"]) if a.band > 0
-              else tune.best_warm_params(ctx, src, rcv, cold_opts, log=log))
-        opts.update(wp, warm=1)
-    log(f"options {opts}")
+              else tune.best_warm_params(ctx, src, rcv, cold_opts, log=log)[0])
+        cands["network"] = dict(cold_opts, **wp, warm=1)
+    if not cands:
+        raise SystemExit(f"--order {a.order} needs a trained network (none was loaded)")
+
+    def whole_calls(o, reps=3):
+        ctx.fim_solve(src, rcv, **o)
+        ts, st = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, _, st = ctx.fim_solve(src, rcv, **o)
+            ts.append(time.perf_counter() - t0)
+        dt = float(np.median(ts))
+        return {"value": len(src) / dt, "unit": "shot-solves/s", "ms_per_step": 1e3 * dt,
+                "visits": st["visits"], "inner_iters": st["inner_iters"], "rounds": st["rounds"],
+                "device_solve_ms": st["solve_ms"], "options": o}
+
+    legs = {k: whole_calls(o) for k, o in cands.items()}
+    for k, v in legs.items():
+        log(f"ordering {k}: {v}")
+    head = max(legs, key=lambda k: legs[k]["value"])
+    opts = cands[head]
+    warm = int(opts.get("warm", 0))
+    log(f"headline ordering {head}, options {opts}")
     for _ in range(a.warmup):
         ctx.fim_solve(src, rcv, **opts)
     times, stats = [], None
@@ -566,18 +604,7 @@ def main():
         log(f"step {k}: {times[-1] * 1e3:.1f} ms, {stats}")
     ms = 1e3 * float(np.median(times))
     value = len(src) / (ms / 1e3)
-
-    # the same batched solver without the network (what the network gains)
-    batched_cold = None
-    if warm and not a.no_cold_leg:
-        ctx.fim_solve(src, rcv, **cold_opts)
-        t0 = time.perf_counter()
-        _, _, cst = ctx.fim_solve(src, rcv, **cold_opts)
-        cdt = time.perf_counter() - t0
-        batched_cold = {"value": len(src) / cdt, "unit": "shot-solves/s", "ms_per_step": 1e3 * cdt,
-                        "visits": cst["visits"], "inner_iters": cst["inner_iters"],
-                        "rounds": cst["rounds"]}
-        log(f"batched cold: {batched_cold}")
+    batched_cold = legs.get("none")
 
     # plain FIM: same kernels, cold start, one shot at a time, its own tuned
     # launch shape (bounded sample)
@@ -633,11 +660,17 @@ def main():
                    "shots": len(src), "receivers": len(rcv), "block": B,
                    "batch": a.batch, "grid": params["grid"],
                    "max_inner": params["max_inner"], "warm_start": bool(warm),
-                   "options": opts},
+                   "ordering": head, "options": opts},
         "plain_fim": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "batched_cold_fim": batched_cold,
-        "network_gain": (value / batched_cold["value"]) if batched_cold else None,
+        # every ordering candidate timed the same way (median of 3 whole
+        # calls on all shots); the fastest is the headline
+        "ordering": legs,
+        "network_gain": (legs["network"]["value"] / legs["none"]["value"]
+                         if "network" in legs and "none" in legs else None),
+        "network_vs_arrival_order": (legs["network"]["value"] / legs["arrival"]["value"]
+                                     if "network" in legs and "arrival" in legs else None),
         "fim_stats_per_step": stats,
         "roofline": roof,
         # every shot must at least write its volume once and read the model:
@@ -646,12 +679,12 @@ def main():
                              "shot_solves_per_s_at_8TBps":
                                  HBM_PEAK_GBS * 1e9 / (8 * int(np.prod(case["vel"].shape)))},
     }
-    if batched_cold:
-        out["warm_start"] = {
-            "visit_reduction": batched_cold["visits"] / max(1, stats["visits"]),
-            "iter_reduction": batched_cold["inner_iters"] / max(1, stats["inner_iters"]),
-            "fallbacks": stats["fallbacks"],
-        }
+    if "none" in legs:
+        for k in ("arrival", "network"):
+            if k in legs:
+                legs[k]["visit_reduction_vs_none"] = legs["none"]["visits"] / max(1, legs[k]["visits"])
+                legs[k]["iter_reduction_vs_none"] = (legs["none"]["inner_iters"]
+                                                     / max(1, legs[k]["inner_iters"]))
 
     if not a.no_cpu:
         threads = a.cpu_threads or host_threads()

@@ -473,7 +473,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             }
             tile[ti] = hv;
         }
-        int it = 0;
+        int it = 0, zflip = 0;
         if constexpr (HYP) {
             tile[tcol] = vz[0];
             tile[tcol + 900] = vz[9];
@@ -498,24 +498,35 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             const bool fx = mxm != mxp ? mxm < mxp : (bx * 8 + 3.5f) * A.h >= sp[0];
             const bool fy = mym != myp ? mym < myp : (by * 8 + 3.5f) * A.h >= sp[1];
             const bool fz = mzm != mzp ? mzm < mzp : (bz * 8 + 3.5f) * A.h >= sp[2];
-            const int xl = fx ? tx : 7 - tx, yl = fy ? ty : 7 - ty;
-            const int zmax = A.n4z - bz * 8;  // cells at or past it are outside the model
+            const int s0 = (fx ? tx : 7 - tx) + (fy ? ty : 7 - ty);
+            // the lane's cell on plane d is z' = d - s0, i.e. column level
+            // k(d) = kb + ks * d: tile and h*s addresses are linear in d.  It
+            // updates on the planes d in [dlo, dlo + span]: 0 <= z' < 8 and
+            // the cell inside the model (k < zmax; the column inside, vxy).
+            const int zmax = A.n4z - bz * 8;  // >= 1: the block holds model cells
+            const int ks = fz ? 1 : -1, kb = fz ? -s0 : 7 + s0;
+            const int zlo = fz ? 0 : max(0, 8 - zmax), zhi = fz ? min(7, zmax - 1) : 7;
+            const int dlo = vxy ? s0 + zlo : 64;
+            const unsigned span = (unsigned)(zhi - zlo);
+            const int i0 = tcol + 100 * (kb + 1), h0 = lane + 64 * kb;
+            // (indexed from the LDS arrays themselves: 32-bit addresses)
+            const int w = threadIdx.x >> 6;
 #pragma unroll
             for (int d = 0; d < 22; d++) {
-                const int zl = d - xl - yl;
-                if (zl >= 0 && zl < 8) {
-                    const int k = fz ? zl : 7 - zl;
-                    const int i = tcol + 100 * (k + 1);
-                    const float own = tile[i];
-                    const float a = fminf(tile[i - 1], tile[i + 1]);
-                    const float bb = fminf(tile[i - 10], tile[i + 10]);
-                    const float c = fminf(tile[i - 100], tile[i + 100]);
-                    const float hk = hsl[lane + 64 * k];
+                if ((unsigned)(d - dlo) <= span) {
+                    const int i = i0 + 100 * ks * d;
+                    const float own = tiles[w][i];
+                    const float a = fminf(tiles[w][i - 1], tiles[w][i + 1]);
+                    const float bb = fminf(tiles[w][i - 10], tiles[w][i + 10]);
+                    const float c = fminf(tiles[w][i - 100], tiles[w][i + 100]);
+                    const float hk = hss[w][h0 + 64 * ks * d];
                     const float t = godunov(a, bb, c, hk, hk * hk);
-                    if (vxy && k < zmax) tile[i] = fminf(own, t);
+                    tiles[w][i] = fminf(own, t);
                 }
                 wave_sync();
             }
+            // the z sweeps start in the pass's z direction
+            zflip = fz ? 1 : 0;
 #pragma unroll
             for (int k = 0; k < 8; k++) vz[k + 1] = tile[tcol + 100 * (k + 1)];
             it = 1;
@@ -551,7 +562,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
                 wave_sync();
                 if (__ballot(ch)) dirty |= ((7u << k) >> 1) & 0xFFu;
             };
-            if ((it & 1) == 0) {
+            if (((it + zflip) & 1) == 0) {
                 #pragma unroll
                 for (int k = 0; k < 8; k++) plane(k);
             } else {

@@ -1,9 +1,11 @@
-"""Autotuner for the batched FIM launch shape.
+"""Autotuner for the batched FIM.
 
 Chooses the block edge (4 or 8), the number of waves per round launch
 (resident waves per CU; each wave walks its own share of the active list)
 and the inner-iteration cap by timing the device solve (HIP events inside the
-library) on a sample of the caller's shots.  Results are cached per model
+library) on a sample of the caller's shots, then the visit ordering (none,
+the solve's own tentative arrivals, the warm-start network's predictions) and
+its band by timing whole calls.  Launch-shape results are cached per model
 shape and shot count for the life of the process.
 """
 from __future__ import annotations
@@ -64,7 +66,8 @@ def load_surrogate(ctx, case, log=None):
 
 def best_warm_params(ctx, src, rcv, cold, sample=256, log=None):
     """Band of the predicted-arrival ordering and inner cap for warm solves,
-    timed over the whole call (network inference included)."""
+    timed over the whole call (network inference included).  Returns
+    (options, ms)."""
     n = min(sample, len(src))
     idx = np.linspace(0, len(src) - 1, n).round().astype(int)
     sub = np.ascontiguousarray(src[idx])
@@ -85,4 +88,27 @@ def best_warm_params(ctx, src, rcv, cold, sample=256, log=None):
                 f"{st['fallbacks']} fallbacks")
         if ms < best_ms:
             best, best_ms = dict(band=f * unit, max_inner=mi, grid=g), ms
-    return best
+    return best, best_ms
+
+
+def best_order_params(ctx, src, rcv, cold, sample=256, log=None, factors=(0.6, 0.9, 1.5)):
+    """Band of the tentative-arrival ordering (order = 1: each round visits,
+    per shot, the scheduled blocks whose arrival lower bound lies within the
+    band of the shot's earliest), timed over the whole call; the band unit is
+    one block crossing at 1500 m/s.  Returns (options, ms)."""
+    n = min(sample, len(src))
+    idx = np.linspace(0, len(src) - 1, n).round().astype(int)
+    sub = np.ascontiguousarray(src[idx])
+    unit = cold["block"] * ctx.spacing[0] / 1500.0
+    best, best_ms = None, float("inf")
+    for f in factors:
+        o = dict(cold, batch=n, order=1, band=f * unit)
+        ctx.fim_solve(sub[:2], rcv, **dict(o, batch=2))
+        _, _, st = ctx.fim_solve(sub, rcv, **o)
+        ms = st["total_ms"]
+        if log:
+            log(f"tune order 1 band {f * unit:.3f}s: {ms:.2f} ms (solve {st['solve_ms']:.2f}) "
+                f"for {n} shots, {st['visits']} visits, {st['inner_iters']} inner iterations")
+        if ms < best_ms:
+            best, best_ms = dict(order=1, band=f * unit), ms
+    return best, best_ms
