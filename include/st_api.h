@@ -253,6 +253,12 @@ int st_surrogate_eval(st_ctx *ctx, const float *X, int ns, const float *target,
 /* training set on the current model grid, solved by the plain FIM */
 int st_surrogate_make_data(st_ctx *ctx, int n_models, int shots_per_model,
                            float zmax_src, unsigned seed, float grad_fraction);
+/* Weight of under-predictions (prediction below the target) in the
+ * training and evaluation loss, >= 1 (1 = plain MSE, the default; reset by
+ * st_surrogate_init).  A large
+ * weight trains the network towards an upper bound of the travel times,
+ * which the (1+delta) value warm start needs. */
+int st_surrogate_loss(st_ctx *ctx, float under_weight);
 /* Adam; history (nullable): per epoch [train loss, validation loss] */
 int st_surrogate_train(st_ctx *ctx, int epochs, int batch, float lr, int n_val,
                        unsigned seed, float *history);

@@ -107,6 +107,7 @@ SIGNATURES = [
                                          c_float_p]),
     ("st_surrogate_make_data", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_float, ctypes.c_uint, ctypes.c_float]),
+    ("st_surrogate_loss", ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
     ("st_surrogate_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_float, ctypes.c_int, ctypes.c_uint, c_float_p]),
     ("st_surrogate_save", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),

@@ -382,6 +382,10 @@ class Context:
         self._check(self.lib.st_surrogate_make_data(self.h, n_models, shots_per_model, zmax_src,
                                                     seed, grad_fraction))
 
+    def surrogate_loss(self, under_weight):
+        """Weight of under-predictions in the training loss (1 = plain MSE)."""
+        self._check(self.lib.st_surrogate_loss(self.h, float(under_weight)))
+
     def surrogate_train(self, epochs, batch=8, lr=1e-3, n_val=64, seed=0):
         hist = np.zeros((epochs, 2), np.float32)
         self._check(self.lib.st_surrogate_train(self.h, epochs, batch, lr, n_val, seed, _ptr(hist)))

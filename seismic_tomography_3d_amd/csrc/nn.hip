@@ -562,17 +562,20 @@ __global__ void bias_grad(const float *__restrict__ dY, float *__restrict__ db, 
     if (threadIdx.x == 0) atomicAdd(&db[c], sh[0]);
 }
 
-// weighted MSE: loss = sum m (p - t)^2 / sum m; grad = 2 m (p - t) / sum m
+// weighted MSE with under-predictions weighted wu: w = m (p < t ? wu : 1),
+// loss = sum w (p - t)^2 / sum m; grad = 2 w (p - t) / sum m.  wu > 1 pulls
+// the prediction towards an upper bound of the target (an expectile).
 __global__ void mse(const float *__restrict__ p, const float *__restrict__ t,
                     const float *__restrict__ m, float *__restrict__ grad, double *__restrict__ acc,
-                    long long n) {
+                    long long n, float wu) {
     __shared__ double sl[256], sw[256];
     double l = 0.0, ws = 0.0;
     GRID_STRIDE(i, n) {
         const float d = p[i] - t[i];
-        l += (double)m[i] * d * d;
+        const float w = d < 0.f ? m[i] * wu : m[i];
+        l += (double)w * d * d;
         ws += m[i];
-        grad[i] = 2.f * m[i] * d;  // scaled by 1/sum m afterwards
+        grad[i] = 2.f * w * d;  // scaled by 1/sum m afterwards
     }
     sl[threadIdx.x] = l;
     sw[threadIdx.x] = ws;
@@ -657,9 +660,9 @@ void bias_gradient(hipStream_t st, const float *dY, float *db, long long rows, i
     hipLaunchKernelGGL(bias_grad, dim3(64, C), dim3(256), 0, st, dY, db, rows, C);
 }
 void mse_loss(hipStream_t st, const float *p, const float *t, const float *m, float *grad,
-              double *acc, long long n) {
+              double *acc, long long n, float under_weight) {
     hipLaunchKernelGGL(mse, dim3(blocks_for(n) < 1024 ? blocks_for(n) : 1024), dim3(256), 0, st, p,
-                       t, m, grad, acc, n);
+                       t, m, grad, acc, n, under_weight);
     hipLaunchKernelGGL(scale_by_inv, dim3(blocks_for(n)), dim3(256), 0, st, grad, acc, n);
 }
 void adam_step(hipStream_t st, float *w, const float *g, float *m, float *v, unsigned short *w16,

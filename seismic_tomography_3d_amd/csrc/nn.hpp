@@ -45,7 +45,7 @@ void relu_backward(hipStream_t st, float *dY, const float *Y, unsigned short *d1
                    bool relu);
 void bias_gradient(hipStream_t st, const float *dY, float *db, long long rows, int C);
 void mse_loss(hipStream_t st, const float *p, const float *t, const float *m, float *grad,
-              double *acc, long long n);
+              double *acc, long long n, float under_weight);
 void adam_step(hipStream_t st, float *w, const float *g, float *m, float *v, unsigned short *w16,
                long long n, float lr, int step);
 void bf16_copy(hipStream_t st, const float *x, unsigned short *y, long long n);

@@ -35,10 +35,12 @@ def attach(ctx, case=None, path=None, log=None):
 
 
 def train(ctx, n_models=2000, spm=4, epochs=20, batch=8, lr=2e-3, n_val=256, seed=5,
-          nc=32, log=print):
+          nc=32, under_weight=1.0, log=print):
     """Generates the training set on the device and runs Adam epoch by epoch
-    with a cosine learning-rate decay (lr -> lr/20)."""
+    with a cosine learning-rate decay (lr -> lr/20).  under_weight > 1 weights
+    under-predictions in the loss, training towards an upper bound."""
     ctx.surrogate_init(nc, seed=seed)
+    ctx.surrogate_loss(under_weight)
     t0 = time.time()
     ctx.surrogate_make_data(n_models, shots_per_model=spm, zmax_src=2000.0, seed=seed)
     t1 = time.time()
@@ -63,12 +65,14 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--history", default=None)
+    ap.add_argument("--under-weight", type=float, default=1.0,
+                    help="loss weight of under-predictions (> 1 trains towards an upper bound)")
     a = ap.parse_args()
     case = synth.batched_case(1)
     ctx = Context(0)
     ctx.set_model(1.0 / case["vel"], case["h"])
     hist = train(ctx, n_models=a.models, spm=a.spm, epochs=a.epochs, batch=a.batch, lr=a.lr,
-                 log=lambda m: print(m, flush=True))
+                 under_weight=a.under_weight, log=lambda m: print(m, flush=True))
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     ctx.surrogate_save(a.out)
     print(f"saved {a.out}")
