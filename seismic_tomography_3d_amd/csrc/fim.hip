@@ -890,6 +890,73 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
     for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
     if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&resmax[s], __float_as_uint(r));
 }
+// The same judge for 8^3 blocks, one wave per (shot, block): the block and
+// its six face halos are staged as a visit stages them (z column per lane,
+// x / y neighbours through an LDS tile), so each cell's neighbours come
+// from registers or LDS instead of six address computations and scattered
+// loads per cell.
+__global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
+    const float *__restrict__ T, const float *__restrict__ S, const float *__restrict__ src,
+    int nshot, int nbx, int nby, int nbz, long long nblk, int n4x, int n4y, int n4z,
+    long long vol, float dx, float dy, float dz, unsigned *__restrict__ resmax) {
+    __shared__ float tiles[kWaves][1000];  // (8+2)^3, index x + 10 y + 100 z
+    const int lane = threadIdx.x & 63;
+    float *tile = tiles[threadIdx.x >> 6];
+    const long long item = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (item >= (long long)nshot * nblk) return;  // wave-uniform
+    const int shot = (int)(item / nblk);
+    const long long b = item - (long long)shot * nblk;
+    const int bx = (int)(b % nbx);
+    const long long rest = b / nbx;
+    const int by = (int)(rest % nby), bz = (int)(rest / nby);
+    const float *Ts = T + (long long)shot * vol;
+    const long long base = b * 512, nbxy = (long long)nbx * nby;
+    const int tx = lane & 7, ty = lane >> 3;
+    const int tcol = (tx + 1) + 10 * (ty + 1);
+    float v[10];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        v[k + 1] = Ts[base + lane + 64 * k];
+        tile[tcol + 100 * (k + 1)] = v[k + 1];
+    }
+    v[0] = (bz > 0) ? Ts[base - 512 * nbxy + lane + 448] : kInf;
+    v[9] = (bz < nbz - 1) ? Ts[base + 512 * nbxy + lane] : kInf;
+    for (int hid = lane; hid < 256; hid += 64) {
+        const int f = hid >> 6, k = hid & 63, u = k & 7, w = k >> 3;
+        int ti;
+        float hv = kInf;
+        switch (f) {
+        case 0: ti = 10 * (u + 1) + 100 * (w + 1);
+                if (bx > 0) hv = Ts[base - 512 + 7 + 8 * u + 64 * w]; break;
+        case 1: ti = 9 + 10 * (u + 1) + 100 * (w + 1);
+                if (bx < nbx - 1) hv = Ts[base + 512 + 8 * u + 64 * w]; break;
+        case 2: ti = (u + 1) + 100 * (w + 1);
+                if (by > 0) hv = Ts[base - 512LL * nbx + u + 56 + 64 * w]; break;
+        default: ti = (u + 1) + 90 + 100 * (w + 1);
+                if (by < nby - 1) hv = Ts[base + 512LL * nbx + u + 64 * w]; break;
+        }
+        tile[ti] = hv;
+    }
+    wave_sync();
+    const int ix = (int)(src[3 * shot] / dx), iy = (int)(src[3 * shot + 1] / dy),
+              iz = (int)(src[3 * shot + 2] / dz);
+    const int x = bx * 8 + tx, y = by * 8 + ty;
+    float r = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int z = bz * 8 + k;
+        if (x < n4x && y < n4y && z < n4z && !(x == ix && y == iy && z == iz)) {
+            const int ti = tcol + 100 * (k + 1);
+            const float a = fminf(tile[ti - 1], tile[ti + 1]);
+            const float bb = fminf(tile[ti - 10], tile[ti + 10]);
+            const float c = fminf(v[k], v[k + 2]);
+            const float hs = dx * S[base + lane + 64 * k];
+            r = fmaxf(r, godunov(a, bb, c, hs, hs * hs) - v[k + 1]);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
+    if (lane == 0 && r > 0.0f) atomicMax(&resmax[shot], __float_as_uint(r));
+}
 }  // namespace st
 
 // Sums the per-wave statistics slots into out[0..2] (one workgroup).
@@ -1193,6 +1260,18 @@ void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res) {
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     ctx->judge.reserve(sizeof(unsigned) * std::max(nshot, 1));
     ST_HIP(hipMemsetAsync(ctx->judge.ptr, 0, sizeof(unsigned) * nshot, ctx->stream));
+    if (B == 8) {
+        const long long items = (long long)nshot * g.nblk;
+        hipLaunchKernelGGL(judge_blocks8, dim3((unsigned)((items + kWaves - 1) / kWaves)),
+                           dim3(64 * kWaves), 0, ctx->stream, ctx->T.as<float>(), block_model(ctx, B),
+                           dsrc, nshot, g.nbx, g.nby, g.nbz, g.nblk, g.n4x, g.n4y, g.n4z, g.vol,
+                           ctx->dx, ctx->dy, ctx->dz, ctx->judge.as<unsigned>());
+        ST_HIP(hipGetLastError());
+        ST_HIP(hipMemcpyAsync(res, ctx->judge.ptr, sizeof(float) * nshot, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        ST_HIP(hipStreamSynchronize(ctx->stream));
+        return;
+    }
     const long long n = (long long)nshot * g.vol;
     hipLaunchKernelGGL(judge_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->T.as<float>(), block_model(ctx, B), dsrc, nshot, B, g.nbx, g.nby, g.n4x,
