@@ -676,13 +676,15 @@ __global__ void pack_model(const float *__restrict__ s, float *__restrict__ S, i
 }
 
 __global__ void fill_inf(float4 *__restrict__ T, long long n4) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n4) T[i] = make_float4(kInf, kInf, kInf, kInf);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (long long)gridDim.x * blockDim.x)
+        T[i] = make_float4(kInf, kInf, kInf, kInf);
 }
 
 __global__ void fill_u32(unsigned *__restrict__ p, long long n, unsigned v) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // apply_source_time (block_FIM.cpp:295-369): the source's grid node gets
@@ -724,13 +726,14 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
 __global__ void extract_volume(const float *__restrict__ T, float *__restrict__ out, int nx, int ny,
                                int nz, int B, int nbx, int nby, long long vol, int nshot) {
     const long long npts = (long long)nx * ny * nz;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts * nshot) return;
-    const long long sh = i / npts, c = i - sh * npts;
-    const int y = (int)(c / ((long long)nx * nz));
-    const long long r = c - (long long)y * nx * nz;
-    const int x = (int)(r / nz), z = (int)(r - (long long)x * nz);
-    out[i] = T[sh * vol + blk_addr(x, y, z, B, nbx, nby)];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npts * nshot;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long sh = i / npts, c = i - sh * npts;
+        const int y = (int)(c / ((long long)nx * nz));
+        const long long r = c - (long long)y * nx * nz;
+        const int x = (int)(r / nz), z = (int)(r - (long long)x * nz);
+        out[i] = T[sh * vol + blk_addr(x, y, z, B, nbx, nby)];
+    }
 }
 
 
@@ -779,8 +782,9 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
                                                 int nby, long long nblk, long long vol, int nx,
                                                 int ny, int nz, float dx, float dy, float dz,
                                                 float scale, float r_cold, int write_values,
-                                                Pred pred) {
-    const long long w = blockIdx.x;
+                                                int nshot, Pred pred) {
+    // one (shot, block) per workgroup and step of the grid stride
+    for (long long w = blockIdx.x; w < (long long)nshot * nblk; w += gridDim.x) {
     const int s = (int)(w / nblk);
     const long long b = w - (long long)s * nblk;
     const int lane = threadIdx.x;
@@ -810,6 +814,7 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
     kmin = wave_min(kmin);
     // no usable prediction in the block: schedule it as early as possible
     if (lane == 0) keys[s * nblk + b] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
+    }
 }
 
 // Keys only (ordering-only warm starts): one thread per (shot, block), the
@@ -821,8 +826,8 @@ template <class Pred>
 __global__ void warm_keys(float *__restrict__ keys, const float *__restrict__ src, int nshot, int B,
                           int nbx, int nby, long long nblk, int nx, int ny, int nz, float dx,
                           float dy, float dz, Pred pred) {
-    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= (long long)nshot * nblk) return;
+    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < (long long)nshot * nblk;
+         w += (long long)gridDim.x * blockDim.x) {
     const int s = (int)(w / nblk);
     const long long b = w - (long long)s * nblk;
     const int bx = (int)(b % nbx);
@@ -847,6 +852,7 @@ __global__ void warm_keys(float *__restrict__ keys, const float *__restrict__ sr
         }
     }
     keys[w] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
+    }
 }
 
 // Judge: largest residual G(T) - T over the domain cells of each shot
@@ -860,8 +866,10 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
                                const float *__restrict__ src, int nshot, int B, int nbx, int nby,
                                int n4x, int n4y, int n4z, long long vol, float dx, float dy,
                                float dz, unsigned *__restrict__ resmax) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n = (long long)nshot * vol;
+    // the launch covers whole waves: every step of the stride keeps them whole
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i - (threadIdx.x & 63) < n;
+         i += (long long)gridDim.x * blockDim.x) {
     const long long iw = i < n ? i : n - 1;  // tail lanes repeat the last cell
     const int s = (int)(iw / vol);
     const long long a = iw - (long long)s * vol;
@@ -889,6 +897,7 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
     }
     for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
     if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&resmax[s], __float_as_uint(r));
+    }
 }
 // The same judge for 8^3 blocks, one wave per (shot, block): the block and
 // its six face halos are staged as a visit stages them (z column per lane,
@@ -902,8 +911,8 @@ __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
     __shared__ float tiles[kWaves][1000];  // (8+2)^3, index x + 10 y + 100 z
     const int lane = threadIdx.x & 63;
     float *tile = tiles[threadIdx.x >> 6];
-    const long long item = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (item >= (long long)nshot * nblk) return;  // wave-uniform
+    for (long long item = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6);
+         item < (long long)nshot * nblk; item += (long long)gridDim.x * kWaves) {  // wave-uniform
     const int shot = (int)(item / nblk);
     const long long b = item - (long long)shot * nblk;
     const int bx = (int)(b % nbx);
@@ -956,6 +965,8 @@ __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
     }
     for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
     if (lane == 0 && r > 0.0f) atomicMax(&resmax[shot], __float_as_uint(r));
+    wave_sync();  // the tile is restaged for the next item
+    }
 }
 }  // namespace st
 
@@ -1058,13 +1069,13 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     if (tentative) {
         const long long n = 2LL * nshot * g.nblk;
         tk = bf.tk;
-        hipLaunchKernelGGL(fill_u32, dim3((unsigned)((n + tpb - 1) / tpb)), dim3(tpb), 0,
-                           ctx->stream, tk, n, kInfBits);
+        hipLaunchKernelGGL(fill_u32, dim3(stride_grid(n, tpb)), dim3(tpb), 0, ctx->stream, tk, n,
+                           kInfBits);
     }
     if (!warm_loaded) {
         const long long n4 = (long long)nshot * g.vol / 4;
-        hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + tpb - 1) / tpb)), dim3(tpb), 0,
-                           ctx->stream, reinterpret_cast<float4 *>(bf.T), n4);
+        hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, tpb)), dim3(tpb), 0, ctx->stream,
+                           reinterpret_cast<float4 *>(bf.T), n4);
     }
     ST_HIP(hipMemsetAsync(bf.flags, 0, sizeof(unsigned) * g.nblk * nshot, ctx->stream));
     hipLaunchKernelGGL(seed_sources, dim3((nshot + 63) / 64), dim3(64), 0, ctx->stream, dsrc,
@@ -1213,19 +1224,19 @@ void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nsho
     FinePred fp{dpred, ctx->nx, ctx->nz, (long long)ctx->nx * ctx->ny * ctx->nz};
     if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
-        hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, 256)), dim3(256), 0, ctx->stream,
                            ctx->T.as<float4>(), n4);
         const long long nk = (long long)nshot * g.nblk;
-        hipLaunchKernelGGL(warm_keys<FinePred>, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(warm_keys<FinePred>, dim3(stride_grid(nk, 256)), dim3(256), 0,
                            ctx->stream, dkeys, dsrc, nshot, B, g.nbx, g.nby, g.nblk, ctx->nx,
                            ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, fp);
         ST_HIP(hipGetLastError());
         return;
     }
-    hipLaunchKernelGGL(warm_fill<FinePred>, dim3((unsigned)(g.nblk * nshot)), dim3(64), 0,
+    hipLaunchKernelGGL(warm_fill<FinePred>, dim3(stride_grid(g.nblk * nshot, 1)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
                        g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
-                       r_cold, values ? 1 : 0, fp);
+                       r_cold, values ? 1 : 0, nshot, fp);
     ST_HIP(hipGetLastError());
 }
 
@@ -1239,19 +1250,19 @@ void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dm
                   (nc - 1) / ((ctx->nz - 1) * ctx->dz), 1.0f / vref, dmin};
     if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
-        hipLaunchKernelGGL(fill_inf, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, 256)), dim3(256), 0, ctx->stream,
                            ctx->T.as<float4>(), n4);
         const long long nk = (long long)nshot * g.nblk;
-        hipLaunchKernelGGL(warm_keys<CoarsePred>, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(warm_keys<CoarsePred>, dim3(stride_grid(nk, 256)), dim3(256), 0,
                            ctx->stream, dkeys, dsrc, nshot, B, g.nbx, g.nby, g.nblk, ctx->nx,
                            ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, cp);
         ST_HIP(hipGetLastError());
         return;
     }
-    hipLaunchKernelGGL(warm_fill<CoarsePred>, dim3((unsigned)(g.nblk * nshot)), dim3(64), 0,
+    hipLaunchKernelGGL(warm_fill<CoarsePred>, dim3(stride_grid(g.nblk * nshot, 1)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
                        g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
-                       r_cold, values ? 1 : 0, cp);
+                       r_cold, values ? 1 : 0, nshot, cp);
     ST_HIP(hipGetLastError());
 }
 
@@ -1262,7 +1273,7 @@ void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res) {
     ST_HIP(hipMemsetAsync(ctx->judge.ptr, 0, sizeof(unsigned) * nshot, ctx->stream));
     if (B == 8) {
         const long long items = (long long)nshot * g.nblk;
-        hipLaunchKernelGGL(judge_blocks8, dim3((unsigned)((items + kWaves - 1) / kWaves)),
+        hipLaunchKernelGGL(judge_blocks8, dim3(stride_grid(items, kWaves)),
                            dim3(64 * kWaves), 0, ctx->stream, ctx->T.as<float>(), block_model(ctx, B),
                            dsrc, nshot, g.nbx, g.nby, g.nbz, g.nblk, g.n4x, g.n4y, g.n4z, g.vol,
                            ctx->dx, ctx->dy, ctx->dz, ctx->judge.as<unsigned>());
@@ -1273,7 +1284,7 @@ void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res) {
         return;
     }
     const long long n = (long long)nshot * g.vol;
-    hipLaunchKernelGGL(judge_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(judge_residual, dim3(stride_grid(n, 256)), dim3(256), 0, ctx->stream,
                        ctx->T.as<float>(), block_model(ctx, B), dsrc, nshot, B, g.nbx, g.nby, g.n4x,
                        g.n4y, g.n4z, g.vol, ctx->dx, ctx->dy, ctx->dz, ctx->judge.as<unsigned>());
     ST_HIP(hipGetLastError());
@@ -1285,7 +1296,7 @@ void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res) {
 void launch_extract(st_ctx *ctx, float *dout, int nshot, int B) {
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     const long long n = (long long)ctx->nx * ctx->ny * ctx->nz * nshot;
-    hipLaunchKernelGGL(extract_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(extract_volume, dim3(stride_grid(n, 256)), dim3(256), 0, ctx->stream,
                        ctx->T.as<float>(), dout, ctx->nx, ctx->ny, ctx->nz, B, g.nbx, g.nby, g.vol,
                        nshot);
     ST_HIP(hipGetLastError());

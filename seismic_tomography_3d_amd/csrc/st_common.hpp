@@ -70,6 +70,15 @@ inline size_t fim_list_bytes(long long nblk, long long nshot) {
     return sizeof(unsigned) * (size_t)kListSegments * (size_t)nblk * (size_t)std::max(nshot, 1LL);
 }
 
+// Workgroups of a launch over n items, tpb per group, for kernels that walk
+// their items with a grid stride.  A dispatch counts its size in work-items
+// with 32 bits: grids of 2^32 work-items or more wrap around and leave items
+// unvisited, so large launches are capped (2^20 groups) and loop instead.
+inline unsigned stride_grid(long long n, int tpb) {
+    const long long g = (n + tpb - 1) / tpb;
+    return (unsigned)std::max(1LL, std::min(g, 1LL << 20));
+}
+
 // Block-major volume geometry.  The FIM domain is the reference's
 // pad-to-4 extent (n4*); blocks of edge B tile the pad-to-B extent, cells
 // beyond n4* are outside the domain and stay at kInf.

@@ -116,3 +116,30 @@ def test_201x201x101_tomography_iteration(ctx):
     r1 = ctx.residual_norm(dobs, dcal1)
     print(f"201x201x101 misfit {r0:.4f} -> {r1:.4f}")
     assert r1 < r0
+
+
+def test_401x401x201_large_batches_equal_small_ones(ctx):
+    """Shots per batch at 401x401x201 are bounded by HBM, not by the launch
+    sizes: 512 shots hold 70 GB of travel times, and the cold fill of
+    512 x 34.6 M cells is more than 2^32 work-items, which a dispatch cannot
+    count (its grid wraps) -- the fills, judges and extractions walk their
+    items with a capped grid instead.  A 512-shot batch must give the picks
+    of 64-shot batches (each solve within 1e-5 s of the fixed point: 1e-4 s
+    apart at most) with the same work (a shot the wrapped fill left at zero
+    was never visited)."""
+    nx, ny, nz, h = 401, 401, 201, 10.0
+    vel = synth.layered_model(nx, ny, nz, h, n_layers=5, n_anom=3, seed=2)
+    L = (nx - 1) * h
+    sx, sy, sz = synth.regular_geometry((0.05 * L, 0.05 * L), (0.05 * L, 0.95 * L),
+                                        (0.95 * L, 0.05 * L), 32, 16, 0.0)
+    src = np.stack([sx, sy, sz], 1).astype(np.float32)
+    rcv = np.stack(synth.surface_grid(16, 0.02 * L, 0.98 * L, 100.0), 1).astype(np.float32)
+    ctx.set_model(1.0 / vel, h)
+    p64, _, s64 = ctx.fim_solve(src, rcv, batch=64)
+    p512, _, s512 = ctx.fim_solve(src, rcv, batch=512)
+    d = np.abs(p512 - p64).max(axis=1)
+    print(f"512 shots: batch 64 {s64['visits']} visits in {s64['batches']} batches, batch 512 "
+          f"{s512['visits']} visits in {s512['batches']}; max |difference| {d.max():.2e} s")
+    assert s512["batches"] == 1
+    assert np.all(d < 1e-4)
+    assert abs(s512["visits"] - s64["visits"]) < 0.05 * s64["visits"]
