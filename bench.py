@@ -327,7 +327,7 @@ def large_inversion(iters=10, log=None, warm=False):
     on a 32x32 grid, 4,096 receivers on a 64x64 grid at 100 m, picks from
     the true model (5 seeded layers + 3 anomalies, seed 2) by the FIM;
     start = lateral average; 81x81x41 inversion grid (50 m); per iteration
-    a forward pass in chunks of 64 shots (FIM + picks + ray tracing into
+    a forward pass in chunks of 256 shots (FIM + picks + ray tracing into
     the CSR matrix), order-2 Tikhonov CGLS of 30 iterations and the model
     update with the 5-sample Gaussian as one captured graph.  Reports data
     RMS misfit and model RMS error per iteration.  warm: order the FIM
@@ -348,7 +348,9 @@ def large_inversion(iters=10, log=None, warm=False):
     ctx = Context(0)
     from seismic_tomography_3d_amd import tune
     warm = 1 if warm and tune.load_surrogate(ctx, None, log=log) else 0
-    fo = dict(batch=64, warm=warm)
+    # 256 shots per chunk: 35 GB of travel times; the FIM's rounds serve all
+    # of them (0.40 s per pass against 0.44 s in chunks of 64)
+    fo = dict(batch=256, warm=warm)
     ctx.set_model(1.0 / vel_true, h)
     a = time.perf_counter()
     dobs, _, _ = ctx.fim_solve(src, rcv, **fo)
@@ -378,7 +380,7 @@ def large_inversion(iters=10, log=None, warm=False):
     ctx.close()
     return {"workload": "401x401x201 at 10 m, 1,024 shots x 4,096 receivers (4.19 M rays), "
                         "81x81x41 grid, order 2, lambda 1e-2, 30 CGLS iterations, Gaussian 5/2.0, "
-                        "chunks of 64 shots",
+                        "chunks of 256 shots",
             "warm_start": bool(warm), "observed_picks_s": t_obs, "iterations": rows,
             "tomo_dims": list(nt)}
 

@@ -153,7 +153,12 @@ __device__ __forceinline__ int iclampd(int v, int lo, int hi) { return v < lo ? 
 constexpr int kXcd = 8;
 constexpr int kRing = 16;   // runs / illumination additions a lane buffers in LDS
 
-template <class L>
+// SEG = false: one step per iteration, the next step's node loaded one step
+// ahead; SEG = true: node segments (below), one memory round trip per node
+// but lanes whose segments differ in length wait for the longest.  The
+// second wins when many waves per SIMD share the latency, the first when a
+// launch has too few rays to fill the SIMDs.
+template <class L, bool SEG>
 __global__ void ray_trace(RayArgs a, L lay, int per) {
     const long long grp = (long long)(blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
     const long long ray = grp * blockDim.x + threadIdx.x;
@@ -209,6 +214,7 @@ __global__ void ray_trace(RayArgs a, L lay, int per) {
         t[4] = Tp(i, j, k + 1);
         t[5] = Tp(i, j, k - 1);
     };
+    if constexpr (!SEG) {
     // Software-pipelined by one step: the next node's loads are issued as
     // soon as the new position is known, and travel while this step's run
     // and illumination bookkeeping runs (a node past the last step is
@@ -263,6 +269,88 @@ __global__ void ray_trace(RayArgs a, L lay, int per) {
         k = k2;
 #pragma unroll
         for (int q = 0; q < 6; q++) tv[q] = tn[q];
+    }
+    } else {
+    // Node segments.  While a ray stays in one padded node its six values,
+    // hence its step (rayStep * dT / norm per axis), do not change, so the
+    // positions of the whole segment follow from ALU work alone: a first
+    // pass walks them to the step that enters the next node and issues that
+    // node's six loads at once; a second pass replays the same steps (the
+    // same FP32 operations, so the same positions) for the cell runs and
+    // the illumination while the loads travel.  One memory round trip per
+    // node instead of per step.
+    int i, j, k;
+    float tv[6];
+    node(zi, xi, yi, i, j, k);
+    load6(i, j, k, tv);
+    int step = 0;
+    for (;;) {
+        const float dTz = (tv[0] - tv[1]) / (2.0f * a.dz);
+        const float dTx = (tv[2] - tv[3]) / (2.0f * a.dx);
+        const float dTy = (tv[4] - tv[5]) / (2.0f * a.dy);
+        const float norm = sqrtf(dTx * dTx + dTy * dTy + dTz * dTz);
+        if (!(norm > 0.0f) || !isfinite(norm)) { ok = false; atomicOr(a.err, 1u); break; }
+        const float cz = rayStep * dTz / norm, cx = rayStep * dTx / norm, cy = rayStep * dTy / norm;
+        // pass 1: the segment's length m (steps until the node changes)
+        float qz = zi, qx = xi, qy = yi;
+        int m = 0, i2 = i, j2 = j, k2 = k;
+        bool capped = false;
+        for (;;) {
+            if (step + m >= a.max_steps) { capped = true; break; }
+            qz -= cz;
+            qx -= cx;
+            qy -= cy;
+            ++m;
+            node(qz, qx, qy, i2, j2, k2);
+            if (i2 != i || j2 != j || k2 != k) break;
+        }
+        // the next node (in bounds: nodes are clamped); past the step cap
+        // the replay below may still reach the source cell first
+        float tn[6];
+        if (!capped) load6(i2, j2, k2, tn);
+        __builtin_amdgcn_sched_barrier(0);
+        // pass 2: the m steps again, with the bookkeeping of each
+        bool done = false;
+        const long long il = i + (long long)j * a.nz + (long long)k * a.nx * a.nz;
+        for (int u = 0; u < m && !done; u++) {
+            zi -= cz;
+            xi -= cx;
+            yi -= cy;
+            const int im = iclampd((int)(zi / a.dzt), 0, a.nzt - 1);
+            const int jm = iclampd((int)(xi / a.dxt), 0, a.nxt - 1);
+            const int km = iclampd((int)(yi / a.dyt), 0, a.nyt - 1);
+            const int cell = im + jm * a.nzt + km * a.nxt * a.nzt;
+            if (cell != cur) {
+                if (cnt) {
+                    if (nrun + nb >= a.max_runs) { ok = false; atomicOr(a.err, 2u); break; }
+                    run_ring[nb++][lane] = make_int2(cur, cnt);
+                }
+                cur = cell;
+                cnt = 0;
+            }
+            cnt++;
+            // illumination at the pre-step node with the reference's strides
+            // (least_squares.cpp:234); writes outside the volume are dropped
+            if (il != il_cur) {
+                if (a.illum && il_cur >= 0 && il_cur < npts)
+                    il_ring[ni++][lane] = make_int2((int)il_cur, __float_as_int(il_acc));
+                il_cur = il;
+                il_acc = 0.f;
+            }
+            il_acc += rayStep;
+            done = cell == sId;
+            if (nb == kRing || ni == kRing) flush();
+        }
+        if (!ok) break;
+        step += m;
+        if (done) break;
+        if (capped) { ok = false; atomicOr(a.err, 1u); break; }
+        i = i2;
+        j = j2;
+        k = k2;
+#pragma unroll
+        for (int q = 0; q < 6; q++) tv[q] = tn[q];
+    }
     }
     flush();
     if (a.illum && il_cur >= 0 && il_cur < npts) atomicAdd(&a.illum[il_cur], il_acc);
@@ -410,8 +498,13 @@ static void trace_append(st_ctx *ctx, const float *T, long long vol, L lay, cons
     ST_REQUIRE((long long)ctx->nx * ctx->ny * ctx->nz < (1LL << 31), ST_ERR_CAPACITY,
                "model too large for ray tracing");
     const int per = (int)(((nrays + 63) / 64 + kXcd - 1) / kXcd);
-    hipLaunchKernelGGL(ray_trace<L>, dim3((unsigned)(per * kXcd)), dim3(64), 0, ctx->stream, a, lay,
-                       per);
+    // node segments once the rays fill at least two waves per SIMD
+    if ((nrays + 63) / 64 >= 2 * 1024)
+        hipLaunchKernelGGL((ray_trace<L, true>), dim3((unsigned)(per * kXcd)), dim3(64), 0,
+                           ctx->stream, a, lay, per);
+    else
+        hipLaunchKernelGGL((ray_trace<L, false>), dim3((unsigned)(per * kXcd)), dim3(64), 0,
+                           ctx->stream, a, lay, per);
     ST_HIP(hipGetLastError());
     unsigned *herr = static_cast<unsigned *>(ctx->host_pinned) + 200;
     ST_HIP(hipMemcpyAsync(herr, t.err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -61,6 +61,44 @@ def test_raytrace_bit_exact_on_same_volume(ctx):
         assert np.array_equal(ill, oill)
 
 
+def test_raytrace_node_segments_bit_exact_many_rays(ctx):
+    """A launch of 2,048 waves or more traces by node segments (the positions
+    inside one node advance without memory traffic, the next node's values
+    load during the segment's bookkeeping); fewer rays trace step by step.
+    140,000 rays of one shot take the segment path: rows, columns, values
+    and illumination must still equal the oracle's bit for bit."""
+    vel, h, src, _ = _small_tomo()
+    ctx.set_model(1.0 / vel, h)
+    d = 5 * h
+    nt = ctx.tomo_set_grid(d)
+    ny, nx, nz = vel.shape
+    rng = np.random.default_rng(11)
+    n = 140_000
+    rcv = np.stack([rng.uniform(0.0, (nx - 1) * h, n), rng.uniform(0.0, (ny - 1) * h, n),
+                    rng.uniform(0.0, (nz - 1) * h, n)], 1).astype(np.float32)
+    _, T, _ = ctx.fim_solve(src[:1], None, return_T=True)
+    ctx.csr_reset()
+    ill = ctx.raytrace_volume(T[0], src[0], rcv, illum=True)
+    rp, c, v = ctx.csr()
+    o = orc.oracle()
+    Tp = np.zeros((ny + 2, nx + 2, nz + 2), np.float32)
+    o.or_expand_fdm(P(np.ascontiguousarray(T[0])), nx, ny, nz, P(Tp))
+    rx, ry, rz = (np.ascontiguousarray(rcv[:, i]) for i in range(3))
+    cap = 1 << 24
+    rn = np.zeros(n, np.int32)
+    cols = np.zeros(cap, np.int32)
+    vals = np.zeros(cap, np.float32)
+    oill = np.zeros(T[0].shape, np.float32)
+    nnz = o.or_ray_trace_shot(P(Tp), nx, ny, nz, h, h, h, *nt, d, d, d,
+                              *(float(x) for x in src[0]), P(rx), P(ry), P(rz), n, 1 << 20,
+                              P(rn), P(cols), P(vals), cap, P(oill))
+    print(f"{n} rays, {nnz} non-zeros")
+    assert np.array_equal(np.diff(rp), rn)
+    assert np.array_equal(c, cols[:nnz])
+    assert np.array_equal(v, vals[:nnz])
+    assert np.array_equal(ill, oill)
+
+
 def test_cgls_equals_lstsq(ctx):
     rng = np.random.default_rng(0)
     N, M = 60, 15
