@@ -584,6 +584,27 @@ __global__ void col_gather(const long long *__restrict__ rowptr, int nrows,
     tval[k] = vals[e];
 }
 
+// acc += val[k] * x[idx[k]] for k = k0, k0 + S, ... below e, in that order,
+// with four entries' loads (index, value, then the gathered x) in flight at
+// a time instead of one: the same sequence of sums, fewer round trips.
+template <int S>
+__device__ __forceinline__ float strided_dot(const float *__restrict__ val,
+                                             const int *__restrict__ idx,
+                                             const float *__restrict__ x, long long k, long long e,
+                                             float acc) {
+    for (; k + 3 * S < e; k += 4 * S) {
+        const int c0 = idx[k], c1 = idx[k + S], c2 = idx[k + 2 * S], c3 = idx[k + 3 * S];
+        const float v0 = val[k], v1 = val[k + S], v2 = val[k + 2 * S], v3 = val[k + 3 * S];
+        const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc += v0 * x0;
+        acc += v1 * x1;
+        acc += v2 * x2;
+        acc += v3 * x3;
+    }
+    for (; k < e; k += S) acc += val[k] * x[idx[k]];
+    return acc;
+}
+
 // y[i] = sum_e A[i, col[e]] x[col[e]]: 16 lanes per row, fixed reduction tree
 __global__ void spmv_csr(const long long *__restrict__ rowptr, const int *__restrict__ idx,
                          const float *__restrict__ val, const float *__restrict__ x,
@@ -594,7 +615,7 @@ __global__ void spmv_csr(const long long *__restrict__ rowptr, const int *__rest
     float acc = 0.f;
     if (row < nrows) {
         const long long b = rowptr[row], e = rowptr[row + 1];
-        for (long long k = b + l; k < e; k += 16) acc += val[k] * x[idx[k]];
+        acc = strided_dot<16>(val, idx, x, b + l, e, acc);
     }
     for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
     if (row < nrows && l == 0) y[row] = acc;
@@ -640,8 +661,7 @@ __global__ __launch_bounds__(256) void spmv_split(const long long *__restrict__ 
         if (row < nrows) {
             const long long b = rowptr[row], e = rowptr[row + 1];
             mine = e - b <= kLongCol;
-            if (mine)
-                for (long long k = b + l; k < e; k += 16) acc += val[k] * x[idx[k]];
+            if (mine) acc = strided_dot<16>(val, idx, x, b + l, e, acc);
         }
         for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
         if (mine && l == 0) y[row] = acc;
@@ -651,7 +671,7 @@ __global__ __launch_bounds__(256) void spmv_split(const long long *__restrict__ 
     const int row = long_list[blockIdx.x - nshort];
     const long long b = rowptr[row], e = rowptr[row + 1];
     float acc = 0.f;
-    for (long long k = b + threadIdx.x; k < e; k += 256) acc += val[k] * x[idx[k]];
+    acc = strided_dot<256>(val, idx, x, b + threadIdx.x, e, acc);
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
