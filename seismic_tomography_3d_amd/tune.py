@@ -35,7 +35,10 @@ def best_params(ctx, src, rcv, block=0, grid=0, max_inner=0, sample=256, log=Non
     if candidates is None:
         candidates = []
         for B in ([block] if block else [4, 8]):
-            for wpc in ([grid // CUS] if grid else ([16] if B == 4 else [16, 32, 48])):
+            # more waves than fit at once (24 per CU) run in generations:
+            # a CU's slot then works through several waves' shares of the
+            # list, which evens out a round's tail
+            for wpc in ([grid // CUS] if grid else ([16] if B == 4 else [32, 48, 96, 192])):
                 for mi in ([max_inner] if max_inner else ([16] if B == 4 else [16, 24])):
                     candidates.append(dict(block=B, grid=wpc * CUS, max_inner=mi))
     best, best_ms = None, float("inf")
