@@ -270,7 +270,7 @@ int st_ctx_destroy(st_ctx *ctx) {
                     &ctx->pod_newv, &ctx->pod_counts, &ctx->pod_nit, &ctx->pod_temp,
                     &ctx->src, &ctx->rcv,
                     &ctx->picks, &ctx->scratch, &ctx->keys, &ctx->kmin, &ctx->judge, &ctx->pred,
-                    &ctx->tkeys})
+                    &ctx->tkeys, &ctx->xfaces})
         b->release();
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -432,13 +432,16 @@ static void solve_impl(st_ctx *ctx, const float *src_xyz, int nshot, const float
     ST_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t per_shot = sizeof(float) * g.vol + sizeof(unsigned) * g.nblk +
                             2 * fim_list_bytes(g.nblk, 1) +
+                            (B == 8 ? sizeof(float) * 128 * g.nblk : 0) +
                             (warm ? sizeof(float) * g.nblk : 0) +
                             (T_pred ? sizeof(float) * npts : 0) + (T_out ? sizeof(float) * npts : 0);
     const size_t have = free_b + ctx->T.bytes + ctx->flags.bytes + ctx->list[0].bytes +
-                        ctx->list[1].bytes + ctx->keys.bytes + ctx->pred.bytes + ctx->scratch.bytes;
+                        ctx->list[1].bytes + ctx->keys.bytes + ctx->pred.bytes + ctx->scratch.bytes +
+                        ctx->xfaces.bytes;
     while (batch > 1 && per_shot * (size_t)batch > have / 2) batch /= 2;
 
     ctx->T.reserve(sizeof(float) * g.vol * batch);
+    if (B == 8) ctx->xfaces.reserve(sizeof(float) * 128 * g.nblk * batch);
     ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
     ctx->list[0].reserve(fim_list_bytes(g.nblk, batch));
     ctx->list[1].reserve(fim_list_bytes(g.nblk, batch));
@@ -595,6 +598,7 @@ int st_fim_solve_noalloc(st_ctx *ctx, const float *src_xyz, int nshot, const flo
             f.kmin = tentative ? c.take<unsigned>(3 * (size_t)b) : nullptr;
             f.tk = tentative ? c.take<unsigned>(2 * (size_t)b * g.nblk) : nullptr;
             f.wstats = stats ? c.take<unsigned long long>(3 * (size_t)nwaves) : nullptr;
+            f.xf = B == 8 ? c.take<float>(128 * (size_t)b * g.nblk) : nullptr;
             f.graphs = false;
             if (bf) {
                 *bf = f; *S = s_; *src = sr_; *rcv = rc_; *pk = pk_; *range = r_;
@@ -862,6 +866,7 @@ static void forward_impl(st_ctx *ctx, const float *src_xyz, int nshot, const flo
         const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
         const int batch = std::max(1, std::min({o.batch, nshot, 1023}));
         ctx->T.reserve(sizeof(float) * g.vol * batch);
+        if (B == 8) ctx->xfaces.reserve(sizeof(float) * 128 * g.nblk * batch);
         ctx->flags.reserve(sizeof(unsigned) * g.nblk * batch);
         ctx->list[0].reserve(fim_list_bytes(g.nblk, batch));
         ctx->list[1].reserve(fim_list_bytes(g.nblk, batch));

@@ -65,6 +65,11 @@ struct FimArgs {
     // a visit scheduled the block -- a lower bound on the block's arrival
     // known from the data itself.  [2][nshot * nblk]
     unsigned *tk;
+    // 8^3 blocks (nullable): a copy of each block's two x faces, [shot][block]
+    // [face: 0 tx = 0, 1 tx = 7][ty + 8 tz], written with the block, so a
+    // neighbour's x halo is 64 contiguous floats instead of one float in each
+    // 32-byte row of the block
+    float *xf;
 };
 
 constexpr unsigned kInfBits = 0x7F800000u;
@@ -463,9 +468,15 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             float hv = kInf;
             switch (f) {
             case 0: ti = 10 * (u + 1) + 100 * (w + 1);
-                    if (bx > 0) hv = Tsh[base - 512 + 7 + 8 * u + 64 * w]; break;
+                    if (bx > 0)
+                        hv = A.xf ? A.xf[((long long)shot * A.nblk + b - 1) * 128 + 64 + k]
+                                  : Tsh[base - 512 + 7 + 8 * u + 64 * w];
+                    break;
             case 1: ti = 9 + 10 * (u + 1) + 100 * (w + 1);
-                    if (bx < A.nbx - 1) hv = Tsh[base + 512 + 8 * u + 64 * w]; break;
+                    if (bx < A.nbx - 1)
+                        hv = A.xf ? A.xf[((long long)shot * A.nblk + b + 1) * 128 + k]
+                                  : Tsh[base + 512 + 8 * u + 64 * w];
+                    break;
             case 2: ti = (u + 1) + 100 * (w + 1);
                     if (by > 0) hv = Tsh[base - 512LL * A.nbx + u + 56 + 64 * w]; break;
             default: ti = (u + 1) + 90 + 100 * (w + 1);
@@ -613,6 +624,14 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         if (__ballot(anydec)) {
             #pragma unroll
             for (int k = 0; k < 8; k++) Tsh[base + lane + 64 * k] = v[k];
+            if (A.xf) {
+                // the face copies from the tile (it holds the final values):
+                // lane l writes entries 2l, 2l + 1 of the 128, one float2
+                const int f = lane >> 5, q = 2 * (lane & 31);  // q = ty + 8 z
+                const int ti = (f ? 8 : 1) + 10 * ((q & 7) + 1) + 100 * ((q >> 3) + 1);
+                *reinterpret_cast<float2 *>(A.xf + ((long long)shot * A.nblk + b) * 128 + 64 * f + q) =
+                    make_float2(tile[ti], tile[ti + 10]);
+            }
         }
         const bool xm = __ballot(fxm) != 0ull, xp = __ballot(fxp) != 0ull;
         const bool ym = __ballot(fym) != 0ull, yp = __ballot(fyp) != 0ull;
@@ -693,7 +712,7 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
                              float *T, unsigned *flags, unsigned *list0, unsigned *cnt,
                              float dx, float dy, float dz, int B, int nbx, int nby,
                              long long vol, long long nblk, const float *keys, unsigned *kmin,
-                             unsigned *tk) {
+                             unsigned *tk, float *xf) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) {  // the sources form segment 0 of the first list
         for (int k = 0; k < 3 * kSeg; k++) cnt[k * kCntStride] = 0;
@@ -705,8 +724,14 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
     const int ix = (int)(sx / dx), iy = (int)(sy / dy), iz = (int)(sz / dz);
     const long long a = blk_addr(ix, iy, iz, B, nbx, nby);
     const float ex = (float)(ix * dx) - sx, ey = (float)(iy * dy) - sy, ez = (float)(iz * dz) - sz;
-    T[(long long)j * vol + a] = S[a] * sqrtf(ex * ex + ey * ey + ez * ez);
+    const float t0 = S[a] * sqrtf(ex * ex + ey * ey + ez * ez);
+    T[(long long)j * vol + a] = t0;
     const unsigned b = (unsigned)(a / (B * B * B));
+    if (xf && B == 8) {  // the source cell's x-face copy
+        const int tx = ix & 7, ty = iy & 7, tz = iz & 7;
+        if (tx == 0 || tx == 7)
+            xf[((long long)j * nblk + b) * 128 + (tx == 7 ? 64 : 0) + ty + 8 * tz] = t0;
+    }
     flags[(long long)j * nblk + b] = 1u;
     list0[j] = ((unsigned)j << kShotShift) | b;
     if (tk) {  // the source block opens the first list with key 0
@@ -782,7 +807,7 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
                                                 int nby, long long nblk, long long vol, int nx,
                                                 int ny, int nz, float dx, float dy, float dz,
                                                 float scale, float r_cold, int write_values,
-                                                int nshot, Pred pred) {
+                                                int nshot, float *__restrict__ xf, Pred pred) {
     // one (shot, block) per workgroup and step of the grid stride
     for (long long w = blockIdx.x; w < (long long)nshot * nblk; w += gridDim.x) {
     const int s = (int)(w / nblk);
@@ -809,7 +834,14 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
             kmin = fminf(kmin, p);
             if (dist >= r_cold) v = fminf(scale * p, kInf);
         }
-        if (write_values) T[s * vol + b * B3 + t] = v;
+        if (write_values) {
+            T[s * vol + b * B3 + t] = v;
+            if (xf && B == 8) {  // the block's x-face copy
+                const int tx = t & 7;
+                if (tx == 0 || tx == 7)
+                    xf[(s * nblk + b) * 128 + (tx == 7 ? 64 : 0) + ((t >> 3) & 7) + 8 * (t >> 6)] = v;
+            }
+        }
     }
     kmin = wave_min(kmin);
     // no usable prediction in the block: schedule it as early as possible
@@ -1041,6 +1073,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     if (keys || tentative) ctx->kmin.reserve(sizeof(unsigned) * 3 * std::max(nshot, 1));
     if (tentative) ctx->tkeys.reserve(sizeof(unsigned) * 2 * nshot * g.nblk);
     if (stats) ctx->wstats.reserve(sizeof(unsigned long long) * 3 * fim_waves(o));
+    if (o.block == 8) ctx->xfaces.reserve(sizeof(float) * 128 * g.nblk * nshot);
     FimBufs bf;
     bf.S = block_model(ctx, o.block);
     bf.T = ctx->T.as<float>();
@@ -1050,6 +1083,7 @@ void fim_batch(st_ctx *ctx, const float *dsrc, int nshot, const st_fim_opts &o,
     bf.kmin = ctx->kmin.as<unsigned>();
     bf.tk = ctx->tkeys.as<unsigned>();
     bf.wstats = ctx->wstats.as<unsigned long long>();
+    bf.xf = o.block == 8 ? ctx->xfaces.as<float>() : nullptr;
     bf.graphs = true;
     fim_batch_on(ctx, bf, dsrc, nshot, o, stats, warm_loaded, keys, tentative);
 }
@@ -1072,15 +1106,21 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
         hipLaunchKernelGGL(fill_u32, dim3(stride_grid(n, tpb)), dim3(tpb), 0, ctx->stream, tk, n,
                            kInfBits);
     }
+    float *xf = B == 8 ? bf.xf : nullptr;
     if (!warm_loaded) {
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, tpb)), dim3(tpb), 0, ctx->stream,
                            reinterpret_cast<float4 *>(bf.T), n4);
+        if (xf) {  // the face copies of cold blocks (a warm fill writes its own)
+            const long long f4 = (long long)nshot * g.nblk * 32;
+            hipLaunchKernelGGL(fill_inf, dim3(stride_grid(f4, tpb)), dim3(tpb), 0, ctx->stream,
+                               reinterpret_cast<float4 *>(xf), f4);
+        }
     }
     ST_HIP(hipMemsetAsync(bf.flags, 0, sizeof(unsigned) * g.nblk * nshot, ctx->stream));
     hipLaunchKernelGGL(seed_sources, dim3((nshot + 63) / 64), dim3(64), 0, ctx->stream, dsrc,
                        nshot, bf.S, bf.T, bf.flags, bf.list0, cnt, ctx->dx, ctx->dy, ctx->dz, B,
-                       g.nbx, g.nby, g.vol, g.nblk, keys, bf.kmin, tk);
+                       g.nbx, g.nby, g.vol, g.nblk, keys, bf.kmin, tk, xf);
     ST_HIP(hipGetLastError());
 
     FimArgs A;
@@ -1104,6 +1144,7 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     A.band = o.band;
     A.src = dsrc;
     A.tk = tk;
+    A.xf = xf;
 
     const int nwaves = fim_waves(o);
     const int grid = nwaves / kWaves;
@@ -1155,6 +1196,7 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
         mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
         mix((long long)__builtin_bit_cast(unsigned, A.h));
         mix((long long)(uintptr_t)A.src); mix(hyp ? 1 : 0); mix((long long)(uintptr_t)A.tk);
+        mix((long long)(uintptr_t)A.xf);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) ST_HIP(hipGraphExecDestroy(ctx->graph_exec));
             if (ctx->graph) ST_HIP(hipGraphDestroy(ctx->graph));
@@ -1217,15 +1259,28 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     }
 }
 
+// The context's x-face copies for a warm fill of 8^3 blocks (null otherwise).
+static float *warm_faces(st_ctx *ctx, const BlockGeo &g, int nshot, int B) {
+    if (B != 8) return nullptr;
+    ctx->xfaces.reserve(sizeof(float) * 128 * g.nblk * nshot);
+    return ctx->xfaces.as<float>();
+}
+
 // Warm values + keys from caller volumes (device, nshot x npts, z fastest).
 void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nshot, int B,
                     float delta, float r_cold, bool values, float *dkeys) {
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     FinePred fp{dpred, ctx->nx, ctx->nz, (long long)ctx->nx * ctx->ny * ctx->nz};
+    float *xf = warm_faces(ctx, g, nshot, B);
     if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, 256)), dim3(256), 0, ctx->stream,
                            ctx->T.as<float4>(), n4);
+        if (xf) {
+            const long long f4 = (long long)nshot * g.nblk * 32;
+            hipLaunchKernelGGL(fill_inf, dim3(stride_grid(f4, 256)), dim3(256), 0, ctx->stream,
+                               reinterpret_cast<float4 *>(xf), f4);
+        }
         const long long nk = (long long)nshot * g.nblk;
         hipLaunchKernelGGL(warm_keys<FinePred>, dim3(stride_grid(nk, 256)), dim3(256), 0,
                            ctx->stream, dkeys, dsrc, nshot, B, g.nbx, g.nby, g.nblk, ctx->nx,
@@ -1236,7 +1291,7 @@ void warm_fill_fine(st_ctx *ctx, const float *dpred, const float *dsrc, int nsho
     hipLaunchKernelGGL(warm_fill<FinePred>, dim3(stride_grid(g.nblk * nshot, 1)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
                        g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
-                       r_cold, values ? 1 : 0, nshot, fp);
+                       r_cold, values ? 1 : 0, nshot, xf, fp);
     ST_HIP(hipGetLastError());
 }
 
@@ -1248,10 +1303,16 @@ void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dm
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     CoarsePred cp{dQ, nc, (nc - 1) / ((ctx->nx - 1) * ctx->dx), (nc - 1) / ((ctx->ny - 1) * ctx->dy),
                   (nc - 1) / ((ctx->nz - 1) * ctx->dz), 1.0f / vref, dmin};
+    float *xf = warm_faces(ctx, g, nshot, B);
     if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, 256)), dim3(256), 0, ctx->stream,
                            ctx->T.as<float4>(), n4);
+        if (xf) {
+            const long long f4 = (long long)nshot * g.nblk * 32;
+            hipLaunchKernelGGL(fill_inf, dim3(stride_grid(f4, 256)), dim3(256), 0, ctx->stream,
+                               reinterpret_cast<float4 *>(xf), f4);
+        }
         const long long nk = (long long)nshot * g.nblk;
         hipLaunchKernelGGL(warm_keys<CoarsePred>, dim3(stride_grid(nk, 256)), dim3(256), 0,
                            ctx->stream, dkeys, dsrc, nshot, B, g.nbx, g.nby, g.nblk, ctx->nx,
@@ -1262,7 +1323,7 @@ void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dm
     hipLaunchKernelGGL(warm_fill<CoarsePred>, dim3(stride_grid(g.nblk * nshot, 1)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
                        g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
-                       r_cold, values ? 1 : 0, nshot, cp);
+                       r_cold, values ? 1 : 0, nshot, xf, cp);
     ST_HIP(hipGetLastError());
 }
 

@@ -137,6 +137,8 @@ struct FimBufs {
     unsigned *flags = nullptr, *list0 = nullptr, *list1 = nullptr;
     unsigned *kmin = nullptr, *tk = nullptr;
     unsigned long long *wstats = nullptr;
+    float *xf = nullptr;           // 8^3 blocks: copies of each block's two x faces
+                                   // (nshot * nblk * 128 floats; nullable)
     bool graphs = false;           // the context's round-graph cache may be used
 };
 }  // namespace st
@@ -170,6 +172,7 @@ struct st_ctx {
     st::DevBuf src, rcv, picks, scratch;
     st::DevBuf keys, kmin, judge, pred;   // warm-start state
     st::DevBuf tkeys;                  // arrival-ordering keys (2 x batch x nblk)
+    st::DevBuf xfaces;                 // 8^3 blocks: x-face copies (batch x nblk x 128)
     void *host_pinned = nullptr;       // small pinned staging area
     void *noalloc = nullptr;           // this device's no-allocation scratch array
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
