@@ -411,9 +411,27 @@ __global__ __launch_bounds__(64 * kWaves) void fim_round4(FimArgs A) {
 // ---------------------------------------------------------------------------
 // Six waves per SIMD (80 VGPRs, a few spilled values outside the plane
 // loops) measured 10 % faster than five without spills.
+//
+// Tile layout: cell (x, y, z) of the (8+2)^3 tile (halo at 0 and 9) at word
+// x + 104 y + 10 z - 11.  gfx950's LDS serves a wave64 access from 64 banks:
+// the z sweeps (all lanes on one z: own cell and its x / y neighbours) then
+// touch 64 distinct banks, since x + 104 y for x, y in 1..8 covers each bank
+// once (104 = 40 mod 64), and the diagonal pass, whose lanes sit on
+// different z, averages 0.55 extra cycles per access.  With x + 10 y + 100 z
+// every z-sweep access cost one extra cycle; measured, the layout makes the
+// 8^3 solve 7 % faster.  Only the interior and the six face halos are ever
+// addressed (never an edge or corner of the tile): on them the map is
+// one-to-one and spans words 0..1013.
+constexpr int kTX = 1, kTY = 104, kTZ = 10, kTBase = 11;
+__host__ __device__ __forceinline__ constexpr int tix(int x, int y, int z) {
+    return kTX * x + kTY * y + kTZ * z - kTBase;
+}
+constexpr int kTile8 = tix(8, 9, 8) + 1;
+static_assert(tix(1, 0, 1) == 0 && tix(0, 1, 1) > 0 && tix(1, 1, 0) > 0, "tile base");
+static_assert(tix(9, 8, 8) < kTile8 && tix(8, 8, 9) < kTile8 && kTile8 == 1014, "tile span");
 template <bool HYP>
 __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
-    __shared__ float tiles[kWaves][1000];  // (8+2)^3, index x + 10 y + 100 z
+    __shared__ float tiles[kWaves][kTile8];  // (8+2)^3 at tix(x, y, z)
     __shared__ float hss[HYP ? kWaves : 1][512];  // h * slowness, tx + 8 ty + 64 z
     const int lane = threadIdx.x & 63;
     float *tile = tiles[threadIdx.x >> 6];
@@ -437,7 +455,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
     const unsigned bit_cur = 1u << p, bit_nxt = 1u << (p ^ 1);
 
     const int tx = lane & 7, ty = lane >> 3;
-    const int tcol = (tx + 1) + 10 * (ty + 1);  // tile index of (tx, ty, z=-1)+100
+    const int tcol = tix(tx + 1, ty + 1, 0);  // tile index of the column's z halo below
     const int nbxy = A.nbx * A.nby;
     unsigned long long my_visits = 0, my_iters = 0, my_defer = 0;
 
@@ -457,7 +475,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             vz[k + 1] = Tsh[base + lane + 64 * k];
             hs[k] = A.h * A.S[base + lane + 64 * k];
             v0[k] = vz[k + 1];
-            tile[tcol + 100 * (k + 1)] = vz[k + 1];
+            tile[tcol + kTZ * (k + 1)] = vz[k + 1];
         }
         vz[0] = (bz > 0) ? Tsh[base - 512LL * nbxy + lane + 64 * 7] : kInf;
         vz[9] = (bz < A.nbz - 1) ? Tsh[base + 512LL * nbxy + lane] : kInf;
@@ -467,19 +485,19 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             int ti;
             float hv = kInf;
             switch (f) {
-            case 0: ti = 10 * (u + 1) + 100 * (w + 1);
+            case 0: ti = tix(0, u + 1, w + 1);
                     if (bx > 0)
                         hv = A.xf ? A.xf[((long long)shot * A.nblk + b - 1) * 128 + 64 + k]
                                   : Tsh[base - 512 + 7 + 8 * u + 64 * w];
                     break;
-            case 1: ti = 9 + 10 * (u + 1) + 100 * (w + 1);
+            case 1: ti = tix(9, u + 1, w + 1);
                     if (bx < A.nbx - 1)
                         hv = A.xf ? A.xf[((long long)shot * A.nblk + b + 1) * 128 + k]
                                   : Tsh[base + 512 + 8 * u + 64 * w];
                     break;
-            case 2: ti = (u + 1) + 100 * (w + 1);
+            case 2: ti = tix(u + 1, 0, w + 1);
                     if (by > 0) hv = Tsh[base - 512LL * A.nbx + u + 56 + 64 * w]; break;
-            default: ti = (u + 1) + 90 + 100 * (w + 1);
+            default: ti = tix(u + 1, 9, w + 1);
                     if (by < A.nby - 1) hv = Tsh[base + 512LL * A.nbx + u + 64 * w]; break;
             }
             tile[ti] = hv;
@@ -487,7 +505,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         int it = 0, zflip = 0;
         if constexpr (HYP) {
             tile[tcol] = vz[0];
-            tile[tcol + 900] = vz[9];
+            tile[tcol + 9 * kTZ] = vz[9];
 #pragma unroll
             for (int k = 0; k < 8; k++) hsl[lane + 64 * k] = hs[k];
             wave_sync();
@@ -500,10 +518,10 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
                 return v;
             };
             const int fu = lane & 7, fw = lane >> 3;
-            const float mxm = wmin(tile[10 * (fu + 1) + 100 * (fw + 1)]);
-            const float mxp = wmin(tile[9 + 10 * (fu + 1) + 100 * (fw + 1)]);
-            const float mym = wmin(tile[(fu + 1) + 100 * (fw + 1)]);
-            const float myp = wmin(tile[(fu + 1) + 90 + 100 * (fw + 1)]);
+            const float mxm = wmin(tile[tix(0, fu + 1, fw + 1)]);
+            const float mxp = wmin(tile[tix(9, fu + 1, fw + 1)]);
+            const float mym = wmin(tile[tix(fu + 1, 0, fw + 1)]);
+            const float myp = wmin(tile[tix(fu + 1, 9, fw + 1)]);
             const float mzm = wmin(vz[0]), mzp = wmin(vz[9]);
             const float *sp = A.src + 3 * shot;
             const bool fx = mxm != mxp ? mxm < mxp : (bx * 8 + 3.5f) * A.h >= sp[0];
@@ -519,17 +537,17 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             const int zlo = fz ? 0 : max(0, 8 - zmax), zhi = fz ? min(7, zmax - 1) : 7;
             const int dlo = vxy ? s0 + zlo : 64;
             const unsigned span = (unsigned)(zhi - zlo);
-            const int i0 = tcol + 100 * (kb + 1), h0 = lane + 64 * kb;
+            const int i0 = tcol + kTZ * (kb + 1), h0 = lane + 64 * kb;
             // (indexed from the LDS arrays themselves: 32-bit addresses)
             const int w = threadIdx.x >> 6;
 #pragma unroll
             for (int d = 0; d < 22; d++) {
                 if ((unsigned)(d - dlo) <= span) {
-                    const int i = i0 + 100 * ks * d;
+                    const int i = i0 + kTZ * ks * d;
                     const float own = tiles[w][i];
-                    const float a = fminf(tiles[w][i - 1], tiles[w][i + 1]);
-                    const float bb = fminf(tiles[w][i - 10], tiles[w][i + 10]);
-                    const float c = fminf(tiles[w][i - 100], tiles[w][i + 100]);
+                    const float a = fminf(tiles[w][i - kTX], tiles[w][i + kTX]);
+                    const float bb = fminf(tiles[w][i - kTY], tiles[w][i + kTY]);
+                    const float c = fminf(tiles[w][i - kTZ], tiles[w][i + kTZ]);
                     const float hk = hss[w][h0 + 64 * ks * d];
                     const float t = godunov(a, bb, c, hk, hk * hk);
                     tiles[w][i] = fminf(own, t);
@@ -539,7 +557,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             // the z sweeps start in the pass's z direction
             zflip = fz ? 1 : 0;
 #pragma unroll
-            for (int k = 0; k < 8; k++) vz[k + 1] = tile[tcol + 100 * (k + 1)];
+            for (int k = 0; k < 8; k++) vz[k + 1] = tile[tcol + kTZ * (k + 1)];
             it = 1;
         }
         wave_sync();
@@ -557,9 +575,9 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             auto plane = [&](const int k) {
                 if (!((dirty >> k) & 1u)) return;
                 dirty &= ~(1u << k);
-                const int ti = tcol + 100 * (k + 1);
-                const float a = fminf(tile[ti - 1], tile[ti + 1]);
-                const float bb = fminf(tile[ti - 10], tile[ti + 10]);
+                const int ti = tcol + kTZ * (k + 1);
+                const float a = fminf(tile[ti - kTX], tile[ti + kTX]);
+                const float bb = fminf(tile[ti - kTY], tile[ti + kTY]);
                 const float c = fminf(vz[k], vz[k + 2]);
                 // HYP keeps h*s in LDS (the pass reads it there): fewer live registers
                 const float hk = HYP ? hsl[lane + 64 * k] : hs[k];
@@ -606,9 +624,9 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             const float dl = v0[k] - v[k];
             anydec |= dl > 0.0f;
             const bool c = dl > A.eps;
-            const int ti = tcol + 100 * (k + 1);
-            const bool ax = c && ((tx == 0 && v[k] < tile[ti - 1]) || (tx == 7 && v[k] < tile[ti + 1]));
-            const bool ay = c && ((ty == 0 && v[k] < tile[ti - 10]) || (ty == 7 && v[k] < tile[ti + 10]));
+            const int ti = tcol + kTZ * (k + 1);
+            const bool ax = c && ((tx == 0 && v[k] < tile[ti - kTX]) || (tx == 7 && v[k] < tile[ti + kTX]));
+            const bool ay = c && ((ty == 0 && v[k] < tile[ti - kTY]) || (ty == 7 && v[k] < tile[ti + kTY]));
             fxm |= ax && tx == 0;
             fxp |= ax && tx == 7;
             fym |= ay && ty == 0;
@@ -628,9 +646,9 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
                 // the face copies from the tile (it holds the final values):
                 // lane l writes entries 2l, 2l + 1 of the 128, one float2
                 const int f = lane >> 5, q = 2 * (lane & 31);  // q = ty + 8 z
-                const int ti = (f ? 8 : 1) + 10 * ((q & 7) + 1) + 100 * ((q >> 3) + 1);
+                const int ti = tix(f ? 8 : 1, (q & 7) + 1, (q >> 3) + 1);
                 *reinterpret_cast<float2 *>(A.xf + ((long long)shot * A.nblk + b) * 128 + 64 * f + q) =
-                    make_float2(tile[ti], tile[ti + 10]);
+                    make_float2(tile[ti], tile[ti + kTY]);
             }
         }
         const bool xm = __ballot(fxm) != 0ull, xp = __ballot(fxp) != 0ull;
