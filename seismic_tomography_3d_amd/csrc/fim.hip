@@ -785,11 +785,18 @@ __global__ void extract_volume(const float *__restrict__ T, float *__restrict__ 
 // on the pad cells past the model; per-block key = smallest prediction in
 // the block (the block's predicted first arrival).  One wave per block.
 // ---------------------------------------------------------------------------
+// Predictors of the warm fill: begin() runs once per (shot, block) with the
+// block's first cell and may stage what the block's cells read; operator()
+// gives the predicted travel time of one cell.
 struct FinePred {  // caller volumes, z fastest, one per shot
     const float *P;
     int nx, nz;
     long long npts;
-    __device__ float operator()(int s, int x, int y, int z, float, float, float, float) const {
+    struct Blk {};
+    __device__ Blk begin(int, int, int, int, int, float, float, float, float *) const { return {}; }
+    __device__ Blk direct(int) const { return {}; }
+    __device__ float operator()(const Blk &, int s, int x, int y, int z, float, float, float,
+                                float) const {
         return P[s * npts + z + (long long)x * nz + (long long)y * nx * nz];
     }
 };
@@ -799,7 +806,11 @@ struct CoarsePred {  // network output y on an nc^3 node cube spanning the model
     int nc;
     float ux, uy, uz;    // (nc - 1) / extent per axis
     float inv_vref, dmin;
-    __device__ float operator()(int s, int, int, int, float px, float py, float pz, float dist) const {
+    struct Blk {};
+    __device__ Blk begin(int, int, int, int, int, float, float, float, float *) const { return {}; }
+    __device__ Blk direct(int) const { return {}; }
+    __device__ float operator()(const Blk &, int s, int, int, int, float px, float py, float pz,
+                                float dist) const {
         const float u = px * ux, v = py * uy, w = pz * uz;
         const int i0 = min((int)u, nc - 2), j0 = min((int)v, nc - 2), k0 = min((int)w, nc - 2);
         const float fu = u - i0, fv = v - j0, fw = w - k0;
@@ -836,6 +847,7 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
     const long long rest = b / nbx;
     const int by = (int)(rest % nby), bz = (int)(rest / nby);
     const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+    const auto blk = pred.begin(s, bx * B, by * B, bz * B, lane, dx, dy, dz, nullptr);
     float kmin = kInf;
     for (int t = lane; t < B3; t += 64) {
         const int tx = t % B, ty = (t / B) % B, tz = t / (B * B);
@@ -845,7 +857,7 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
             const float px = x * dx, py = y * dy, pz = z * dz;
             const float ex = px - sx, ey = py - sy, ez = pz - sz;
             const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-            float p = pred(s, x, y, z, px, py, pz, dist);
+            float p = pred(blk, s, x, y, z, px, py, pz, dist);
             // a non-finite prediction (exponent all ones) counts as unreached;
             // tested on the bits, which survive no-NaN code generation
             if ((__float_as_uint(p) & 0x7F800000u) == 0x7F800000u) p = kInf;
@@ -863,6 +875,78 @@ __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__
     }
     kmin = wave_min(kmin);
     // no usable prediction in the block: schedule it as early as possible
+    if (lane == 0) keys[s * nblk + b] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
+    }
+}
+
+// warm_fill for the network's node cube on 8^3 blocks whose cells span under
+// three node spacings along z.  A lane's eight cells form one z column: the
+// x / y interpolation (the c00..c11 and c0 / c1 terms of CoarsePred, the same
+// expressions) is made once per column at the block's five lowest node
+// levels, and each cell only interpolates between two of them in z.
+__global__ __launch_bounds__(64) void warm_fill_coarse8(float *__restrict__ T,
+                                                        float *__restrict__ keys,
+                                                        const float *__restrict__ src, int nbx,
+                                                        int nby, long long nblk, long long vol,
+                                                        int nx, int ny, int nz, float dx, float dy,
+                                                        float dz, float scale, float r_cold,
+                                                        int write_values, int nshot,
+                                                        float *__restrict__ xf, CoarsePred cp) {
+    const int nc = cp.nc;
+    for (long long w = blockIdx.x; w < (long long)nshot * nblk; w += gridDim.x) {
+    const int s = (int)(w / nblk);
+    const long long b = w - (long long)s * nblk;
+    const int lane = threadIdx.x, tx = lane & 7, ty = lane >> 3;
+    const int bx = (int)(b % nbx);
+    const long long rest = b / nbx;
+    const int by = (int)(rest % nby), bz = (int)(rest / nby);
+    const int x = bx * 8 + tx, y = by * 8 + ty;
+    const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
+    const float *q = cp.Q + (long long)s * nc * nc * nc;
+    const float px = x * dx, py = y * dy;
+    const float u = px * cp.ux, v = py * cp.uy;
+    const int i0 = min((int)u, nc - 2), j0 = min((int)v, nc - 2);
+    const float fu = u - i0, fv = v - j0;
+    // lowest node level of the block's cells (w grows with z)
+    const int kb = min((int)((bz * 8 * dz) * cp.uz), nc - 2);
+    auto at = [&](int i, int j, int k) { return q[i + nc * (j + nc * k)]; };
+    float cl[5];
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+        const int k = min(kb + l, nc - 1);
+        const float c0 = at(i0, j0, k) * (1 - fu) + at(i0 + 1, j0, k) * fu;
+        const float c1 = at(i0, j0 + 1, k) * (1 - fu) + at(i0 + 1, j0 + 1, k) * fu;
+        cl[l] = c0 * (1 - fv) + c1 * fv;
+    }
+    const bool inxy = x < nx && y < ny;
+    const float ex = px - sx, ey = py - sy;
+    float kmin = kInf;
+#pragma unroll
+    for (int tz = 0; tz < 8; tz++) {
+        const int z = bz * 8 + tz;
+        float val = kInf;
+        if (inxy && z < nz) {
+            const float pz = z * dz;
+            const float ez = pz - sz;
+            const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+            const float ww = pz * cp.uz;
+            const int k0 = min((int)ww, nc - 2);
+            const float fw = ww - k0;
+            const int l = k0 - kb;  // 0 .. 3
+            const float c0 = l == 0 ? cl[0] : l == 1 ? cl[1] : l == 2 ? cl[2] : cl[3];
+            const float c1 = l == 0 ? cl[1] : l == 1 ? cl[2] : l == 2 ? cl[3] : cl[4];
+            float p = (c0 * (1 - fw) + c1 * fw) * fmaxf(dist, cp.dmin) * cp.inv_vref;
+            if ((__float_as_uint(p) & 0x7F800000u) == 0x7F800000u) p = kInf;
+            kmin = fminf(kmin, p);
+            if (dist >= r_cold) val = fminf(scale * p, kInf);
+        }
+        if (write_values) {
+            T[s * vol + b * 512 + lane + 64 * tz] = val;
+            if (xf && (tx == 0 || tx == 7))
+                xf[(s * nblk + b) * 128 + (tx == 7 ? 64 : 0) + ty + 8 * tz] = val;
+        }
+    }
+    kmin = wave_min(kmin);
     if (lane == 0) keys[s * nblk + b] = kmin < kInf ? fmaxf(kmin, 0.0f) : 0.0f;
     }
 }
@@ -895,7 +979,8 @@ __global__ void warm_keys(float *__restrict__ keys, const float *__restrict__ sr
                 const int x = i == 0 ? x0 : (i == 1 ? (x0 + x1) / 2 : x1);
                 const float px = x * dx, py = y * dy, pz = z * dz;
                 const float ex = px - sx, ey = py - sy, ez = pz - sz;
-                float p = pred(s, x, y, z, px, py, pz, sqrtf(ex * ex + ey * ey + ez * ez));
+                float p = pred(pred.direct(s), s, x, y, z, px, py, pz,
+                               sqrtf(ex * ex + ey * ey + ez * ez));
                 if ((__float_as_uint(p) & 0x7F800000u) == 0x7F800000u) p = kInf;
                 kmin = fminf(kmin, p);
             }
@@ -957,26 +1042,29 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
 __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
     const float *__restrict__ T, const float *__restrict__ S, const float *__restrict__ src,
     int nshot, int nbx, int nby, int nbz, long long nblk, int n4x, int n4y, int n4z,
-    long long vol, float dx, float dy, float dz, unsigned *__restrict__ resmax) {
-    __shared__ float tiles[kWaves][1000];  // (8+2)^3, index x + 10 y + 100 z
+    long long vol, float dx, float dy, float dz, const float *__restrict__ xf,
+    unsigned *__restrict__ resmax) {
+    __shared__ float tiles[kWaves][kTile8];  // the FIM kernel's tile layout
     const int lane = threadIdx.x & 63;
     float *tile = tiles[threadIdx.x >> 6];
+    // block-major: consecutive items are one block of consecutive shots, so
+    // the block's slowness is read from L2 after the first
     for (long long item = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6);
          item < (long long)nshot * nblk; item += (long long)gridDim.x * kWaves) {  // wave-uniform
-    const int shot = (int)(item / nblk);
-    const long long b = item - (long long)shot * nblk;
+    const long long b = item / nshot;
+    const int shot = (int)(item - b * nshot);
     const int bx = (int)(b % nbx);
     const long long rest = b / nbx;
     const int by = (int)(rest % nby), bz = (int)(rest / nby);
     const float *Ts = T + (long long)shot * vol;
     const long long base = b * 512, nbxy = (long long)nbx * nby;
     const int tx = lane & 7, ty = lane >> 3;
-    const int tcol = (tx + 1) + 10 * (ty + 1);
+    const int tcol = tix(tx + 1, ty + 1, 0);
     float v[10];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         v[k + 1] = Ts[base + lane + 64 * k];
-        tile[tcol + 100 * (k + 1)] = v[k + 1];
+        tile[tcol + kTZ * (k + 1)] = v[k + 1];
     }
     v[0] = (bz > 0) ? Ts[base - 512 * nbxy + lane + 448] : kInf;
     v[9] = (bz < nbz - 1) ? Ts[base + 512 * nbxy + lane] : kInf;
@@ -985,13 +1073,19 @@ __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
         int ti;
         float hv = kInf;
         switch (f) {
-        case 0: ti = 10 * (u + 1) + 100 * (w + 1);
-                if (bx > 0) hv = Ts[base - 512 + 7 + 8 * u + 64 * w]; break;
-        case 1: ti = 9 + 10 * (u + 1) + 100 * (w + 1);
-                if (bx < nbx - 1) hv = Ts[base + 512 + 8 * u + 64 * w]; break;
-        case 2: ti = (u + 1) + 100 * (w + 1);
+        case 0: ti = tix(0, u + 1, w + 1);
+                if (bx > 0)
+                    hv = xf ? xf[((long long)shot * nblk + b - 1) * 128 + 64 + k]
+                            : Ts[base - 512 + 7 + 8 * u + 64 * w];
+                break;
+        case 1: ti = tix(9, u + 1, w + 1);
+                if (bx < nbx - 1)
+                    hv = xf ? xf[((long long)shot * nblk + b + 1) * 128 + k]
+                            : Ts[base + 512 + 8 * u + 64 * w];
+                break;
+        case 2: ti = tix(u + 1, 0, w + 1);
                 if (by > 0) hv = Ts[base - 512LL * nbx + u + 56 + 64 * w]; break;
-        default: ti = (u + 1) + 90 + 100 * (w + 1);
+        default: ti = tix(u + 1, 9, w + 1);
                 if (by < nby - 1) hv = Ts[base + 512LL * nbx + u + 64 * w]; break;
         }
         tile[ti] = hv;
@@ -1005,9 +1099,9 @@ __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
     for (int k = 0; k < 8; k++) {
         const int z = bz * 8 + k;
         if (x < n4x && y < n4y && z < n4z && !(x == ix && y == iy && z == iz)) {
-            const int ti = tcol + 100 * (k + 1);
-            const float a = fminf(tile[ti - 1], tile[ti + 1]);
-            const float bb = fminf(tile[ti - 10], tile[ti + 10]);
+            const int ti = tcol + kTZ * (k + 1);
+            const float a = fminf(tile[ti - kTX], tile[ti + kTX]);
+            const float bb = fminf(tile[ti - kTY], tile[ti + kTY]);
             const float c = fminf(v[k], v[k + 2]);
             const float hs = dx * S[base + lane + 64 * k];
             r = fmaxf(r, godunov(a, bb, c, hs, hs * hs) - v[k + 1]);
@@ -1321,6 +1415,10 @@ void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dm
     const BlockGeo g = make_geo(ctx->nx, ctx->ny, ctx->nz, B);
     CoarsePred cp{dQ, nc, (nc - 1) / ((ctx->nx - 1) * ctx->dx), (nc - 1) / ((ctx->ny - 1) * ctx->dy),
                   (nc - 1) / ((ctx->nz - 1) * ctx->dz), 1.0f / vref, dmin};
+    // a block's cells span 7 spacings along z: under three node spacings,
+    // their lower node levels take at most four values (five levels with the
+    // upper ones)
+    const bool columns = B == 8 && 7.0f * ctx->dz * cp.uz < 2.99f;
     float *xf = warm_faces(ctx, g, nshot, B);
     if (!values) {  // cold values, keys from a sampled lattice
         const long long n4 = (long long)nshot * g.vol / 4;
@@ -1338,6 +1436,14 @@ void warm_fill_coarse(st_ctx *ctx, const float *dQ, int nc, float vref, float dm
         ST_HIP(hipGetLastError());
         return;
     }
+    if (columns) {
+        hipLaunchKernelGGL(warm_fill_coarse8, dim3(stride_grid(g.nblk * nshot, 1)), dim3(64), 0,
+                           ctx->stream, ctx->T.as<float>(), dkeys, dsrc, g.nbx, g.nby, g.nblk,
+                           g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz,
+                           1.0f + delta, r_cold, 1, nshot, xf, cp);
+        ST_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(warm_fill<CoarsePred>, dim3(stride_grid(g.nblk * nshot, 1)), dim3(64), 0,
                        ctx->stream, ctx->T.as<float>(), dkeys, dsrc, B, g.nbx, g.nby, g.nblk,
                        g.vol, ctx->nx, ctx->ny, ctx->nz, ctx->dx, ctx->dy, ctx->dz, 1.0f + delta,
@@ -1352,10 +1458,13 @@ void judge_batch(st_ctx *ctx, const float *dsrc, int nshot, int B, float *res) {
     ST_HIP(hipMemsetAsync(ctx->judge.ptr, 0, sizeof(unsigned) * nshot, ctx->stream));
     if (B == 8) {
         const long long items = (long long)nshot * g.nblk;
+        // the solve leaves the x-face copies equal to the volumes' faces
+        const float *xf = ctx->xfaces.bytes >= sizeof(float) * 128 * (size_t)items
+                              ? ctx->xfaces.as<float>() : nullptr;
         hipLaunchKernelGGL(judge_blocks8, dim3(stride_grid(items, kWaves)),
                            dim3(64 * kWaves), 0, ctx->stream, ctx->T.as<float>(), block_model(ctx, B),
                            dsrc, nshot, g.nbx, g.nby, g.nbz, g.nblk, g.n4x, g.n4y, g.n4z, g.vol,
-                           ctx->dx, ctx->dy, ctx->dz, ctx->judge.as<unsigned>());
+                           ctx->dx, ctx->dy, ctx->dz, xf, ctx->judge.as<unsigned>());
         ST_HIP(hipGetLastError());
         ST_HIP(hipMemcpyAsync(res, ctx->judge.ptr, sizeof(float) * nshot, hipMemcpyDeviceToHost,
                               ctx->stream));
