@@ -21,6 +21,7 @@ namespace nn {
 // ---------------------------------------------------------------------------
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 
 __global__ __launch_bounds__(256) void gemm_bf16(GemmArgs g) {
     __shared__ unsigned short As[64][40];  // [m][k]
@@ -108,14 +109,21 @@ void gemm(hipStream_t st, const GemmArgs &g) {
 //
 // One workgroup (4 waves) per BX x 4 x 4 output brick of one sample.  The
 // (BX + KS - 1) x (4 + KS - 1)^2 input halo, all channels, is staged once in
-// LDS as BF16 (row stride CINP + 4 or + 8 halves to spread banks).  M-tile t holds
-// the 16 voxels x = 4 (t >> 2) + (m & 3), y = m >> 2, z = t & 3.  K = tap *
-// CINP + c in steps of 16 (v_mfma_f32_16x16x16_bf16): a lane's four
-// consecutive k share one tap, so its A fragment is one 8-byte LDS read and
-// its B fragment one 8-byte load of the fragment-ordered weights (a wave
-// reads 512 contiguous bytes; n padded to 16 NT, c to CINP, a zero tap
-// absorbing the K padding of CINP = 8).  The B fragments of a wave's next
-// PD tap groups are in flight while the current one computes.
+// LDS as BF16.  M-tile t holds the 16 voxels x = 4 (t >> 2) + (m & 3) and
+// z = m >> 2, y = t & 3 (3^3 layers) or y = m >> 2, z = t & 3 (1^3 layers).
+// K = tap * CINP + c in steps of 32
+// (v_mfma_f32_16x16x32_bf16, lane m + 16 q holding k = 8 q .. 8 q + 7):
+// CINP is a multiple of 8, so a lane's eight k share one tap and its A
+// fragment is one 16-byte LDS read, its B fragment one 16-byte load of the
+// fragment-ordered weights (a wave reads 1 KB contiguous; n padded to 16 NT,
+// zero weights past the last tap).  The B fragments of a wave's next PD
+// steps are in flight while the current one computes.
+//
+// LDS banks (64 x 4 bytes; a 16-byte read is served 16 lanes at a time): the
+// 16 lanes of one q read one channel chunk of the 16 voxels of an m-tile,
+// x = m & 3 plus a row index m >> 2 along z (3^3 layers) or y (1^3 layers).
+// With a voxel stride of an odd number of 16-byte chunks and a row stride of
+// 4 or 12 mod 16 voxels, the 16 voxels fall on 16 distinct 4-bank groups.
 // ---------------------------------------------------------------------------
 template <int KS, int CINP, int NT, int BX, bool VEC, bool IBF, bool OBF>
 __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, int C0, int up0,
@@ -127,23 +135,24 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
     using TI = typename std::conditional<IBF, unsigned short, float>::type;
     const TI *__restrict__ X0 = static_cast<const TI *>(X0v);
     const TI *__restrict__ X1 = static_cast<const TI *>(X1v);
-    constexpr int HF = KS / 2, HX = BX + KS - 1, H = 4 + KS - 1, NH = HX * H * H;
-    // LDS row stride of a halo voxel, in halves: CINP + 8 spreads a half
-    // wave's A-fragment reads (16 voxels x 4 channel quads) evenly over the
-    // banks (measured 3-7 % faster from 48 inputs up); the narrower layers
-    // ran 2-7 % slower with it and keep CINP + 4
-    constexpr int TAPS = KS * KS * KS, CST = CINP >= 48 ? CINP + 8 : CINP + 4;
-    // tap groups of 16-wide k steps: CINP >= 16 -> one tap of CINP / 16
-    // steps; CINP = 8 -> two taps in one step
-    constexpr int GS = CINP >= 16 ? CINP / 16 : 1;
-    constexpr int NG = CINP >= 16 ? TAPS : (TAPS + 1) / 2;
-    static_assert(CINP == 8 || CINP % 16 == 0, "channel padding");
-    // KS = 3 with 4 x 4 x 4 bricks (the wide-input layers): the tap groups
-    // are dealt to the 4 waves (g = w, w + 4, ...), each wave computes all 4
-    // m-tiles over its taps (every weight fragment is read by one wave of
+    constexpr int HF = KS / 2, HX = BX + KS - 1, H = 4 + KS - 1;
+    // ZROW: an m-tile's four voxel rows run along z (halo stride HX H: 36,
+    // 60 or 108 for the 3^3 layers, all 4 or 12 mod 16); otherwise along y
+    // with the halo's x extent padded to 4 or 12 mod 16 (1^3 layers: 16 -> 20)
+    constexpr bool ZROW = KS == 3;
+    constexpr int HXP = ZROW ? HX : (HX + 3) / 4 * 4 + (((HX + 3) / 4) % 2 == 0 ? 4 : 0);
+    constexpr int NH = HXP * H * H;
+    // voxel stride in halves: a multiple of 8 with an odd quotient
+    constexpr int TAPS = KS * KS * KS, CST = (CINP / 8) % 2 == 1 ? CINP : CINP + 8;
+    constexpr int KTOT = TAPS * CINP, NS = (KTOT + 31) / 32;  // 32-wide k steps
+    static_assert(CINP % 8 == 0 && (CST / 8) % 2 == 1, "halo layout");
+    // KS = 3 with 4 x 4 x 4 bricks (the wide-input layers): the k steps are
+    // dealt to the 4 waves (s = w, w + 4, ...), each wave computes all 4
+    // m-tiles over its steps (every weight fragment is read by one wave of
     // the workgroup), and the partial sums meet in LDS in a fixed order.
-    // Otherwise wave w owns m-tiles w and w + 4 over all taps (measured
-    // faster for the narrow 8 x 4 x 4 bricks).
+    // Otherwise wave w owns m-tiles w and w + 4 over all steps (measured
+    // faster for the 8 x 4 x 4 bricks: 1.03 against 1.24 ms for the
+    // 48-input layer at 256 x 32^3).
     constexpr bool KSPLIT = KS > 1 && BX == 4;
     constexpr int MT = KSPLIT ? BX : BX / 4, GSTEP = KSPLIT ? 4 : 1;
     constexpr int NHALO = NH * CST, NPART = KSPLIT ? 1024 * MT * NT : 0;  // halves
@@ -169,7 +178,7 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
     // BF16 halos of the 16-output-channel layers (few registers in use)
     // are loaded in one batch where 24 items per thread cover them (two
     // registers an item); the others in batches of 8
-    constexpr int NI = NH * C4, NPT = (NI + 255) / 256;
+    constexpr int NI = HX * H * H * C4, NPT = (NI + 255) / 256;
     constexpr int SU = (IBF && NT == 1) ? (NPT < 24 ? NPT : 24) : 8;
     for (int base = 0; base < NI; base += 256 * SU) {
         float f[SU][4];      // FP32 inputs
@@ -222,6 +231,8 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
             const int i = base + u * 256 + tid;
             if (i >= NI) continue;
             const int hv = i / C4, c = (i - hv * C4) * 4;
+            const int hx = hv % HX, hr = hv / HX;  // hr = hy + H hz
+            const int lv = hx + HXP * hr;           // LDS voxel
             uint2 w2;
             if constexpr (IBF) {
                 w2 = raw[u];
@@ -229,7 +240,7 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
                 w2.x = (unsigned)f2bf(f[u][0]) | ((unsigned)f2bf(f[u][1]) << 16);
                 w2.y = (unsigned)f2bf(f[u][2]) | ((unsigned)f2bf(f[u][3]) << 16);
             }
-            *reinterpret_cast<uint2 *>(&halo[hv * CST + c]) = w2;
+            *reinterpret_cast<uint2 *>(&halo[lv * CST + c]) = w2;
         }
     }
     __syncthreads();
@@ -240,12 +251,10 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
 #pragma unroll
     for (int t = 0; t < MT; t++) {
         const int mt = KSPLIT ? t : w + 4 * t;
-        const int vx = 4 * (mt >> 2) + (m & 3), vy = m >> 2, vz = mt & 3;
-        hb[t] = vx + HX * (vy + H * vz);
+        const int vx = 4 * (mt >> 2) + (m & 3);
+        const int vy = ZROW ? mt & 3 : m >> 2, vz = ZROW ? m >> 2 : mt & 3;
+        hb[t] = vx + HXP * (vy + H * vz);
     }
-    // lane's tap and channel base within a tap group
-    auto tap_of = [&](int g) { return CINP >= 16 ? g : 2 * g + (q >> 1); };
-    const int cq = CINP >= 16 ? 4 * q : 4 * (q & 1);
 
     f4v acc[MT][NT];
 #pragma unroll
@@ -253,46 +262,42 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
 #pragma unroll
         for (int j = 0; j < NT; j++) acc[t][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    // B fragments of the next PD tap groups in flight: a ring indexed at
-    // compile time (the group loop is unrolled PD times), so no register
-    // copy waits for a load issued in the same iteration
-    constexpr int PDR = 32 / (2 * GS * NT);
+    // B fragments of the next PD steps in flight: a ring indexed at compile
+    // time (the step loop is unrolled PD times), so no register copy waits
+    // for a load issued in the same iteration
+    constexpr int PDR = 32 / (4 * NT);
     constexpr int PD = PDR < 2 ? 2 : (PDR > 8 ? 8 : PDR);
-    s4v bq[PD][GS][NT];
-    auto load_b = [&](int g, s4v (&b)[GS][NT]) {
+    b8v bq[PD][NT];
+    auto load_b = [&](int st, b8v (&b)[NT]) {
 #pragma unroll
-        for (int k = 0; k < GS; k++)
-#pragma unroll
-            for (int j = 0; j < NT; j++)
-                b[k][j] = *reinterpret_cast<const s4v *>(
-                    Wp + ((((long long)g * GS + k) * NT + j) * 64 + lane) * 4);
+        for (int j = 0; j < NT; j++)
+            b[j] = *reinterpret_cast<const b8v *>(Wp + (((long long)st * NT + j) * 64 + lane) * 8);
     };
-    const int g0 = KSPLIT ? w : 0;
+    const int s0 = KSPLIT ? w : 0;
 #pragma unroll
     for (int d = 0; d < PD; d++)
-        if (g0 + d * GSTEP < NG) load_b(g0 + d * GSTEP, bq[d]);
-    for (int gb = g0; gb < NG; gb += PD * GSTEP) {
+        if (s0 + d * GSTEP < NS) load_b(s0 + d * GSTEP, bq[d]);
+    for (int sb = s0; sb < NS; sb += PD * GSTEP) {
 #pragma unroll
         for (int d = 0; d < PD; d++) {
-            const int g = gb + d * GSTEP;
-            if (g >= NG) break;
-            int tap = tap_of(g);
-            if (tap >= TAPS) tap = 0;  // padding tap: its weights are zero
+            const int st = sb + d * GSTEP;
+            if (st >= NS) break;
+            const int kb = 32 * st + 8 * q;
+            int tap = kb / CINP;
+            const int c = kb - tap * CINP;
+            if (tap >= TAPS) tap = 0;  // past the last tap: the weights are zero
             const int tx = tap % KS, ty = (tap / KS) % KS, tz = tap / (KS * KS);
-            const int toff = tx + HX * (ty + H * tz);
+            const int toff = tx + HXP * (ty + H * tz);
+            b8v a[MT];
 #pragma unroll
-            for (int k = 0; k < GS; k++) {
-                s4v a[MT];
+            for (int t = 0; t < MT; t++)
+                a[t] = *reinterpret_cast<const b8v *>(&halo[(hb[t] + toff) * CST + c]);
 #pragma unroll
-                for (int t = 0; t < MT; t++)
-                    a[t] = *reinterpret_cast<const s4v *>(&halo[(hb[t] + toff) * CST + k * 16 + cq]);
+            for (int t = 0; t < MT; t++)
 #pragma unroll
-                for (int t = 0; t < MT; t++)
-#pragma unroll
-                    for (int j = 0; j < NT; j++)
-                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bq[d][k][j], acc[t][j], 0, 0, 0);
-            }
-            if (g + PD * GSTEP < NG) load_b(g + PD * GSTEP, bq[d]);
+                for (int j = 0; j < NT; j++)
+                    acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bq[d][j], acc[t][j], 0, 0, 0);
+            if (st + PD * GSTEP < NS) load_b(st + PD * GSTEP, bq[d]);
         }
     }
 
@@ -339,8 +344,9 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int mo = 4 * q + r;
-                const int x = BX * bx + 4 * (mt >> 2) + (mo & 3), y = 4 * by + (mo >> 2),
-                          z = 4 * bz + (mt & 3);
+                const int x = BX * bx + 4 * (mt >> 2) + (mo & 3),
+                          y = 4 * by + (ZROW ? mt & 3 : mo >> 2),
+                          z = 4 * bz + (ZROW ? mo >> 2 : mt & 3);
                 if (x >= e || y >= e || z >= e) continue;
                 float v = acc[t][j][r] + bn;
                 if (relu) v = fmaxf(v, 0.f);
@@ -354,28 +360,22 @@ __global__ __launch_bounds__(256) void conv3d_fwd(const void *__restrict__ X0v, 
     }
 }
 
-// Wp in MFMA fragment order: fragment (g, k, j) is 64 lanes x 4 BF16,
-// lane = m + 16 q holding W[tap][c .. c+3][n = 16 j + m] with, for cinp >= 16,
-// tap = g and c = 16 k + 4 q (k < cinp / 16), and for cinp = 8 (two taps per
-// group) tap = 2 g + (q >> 1) and c = 4 (q & 1); zero where tap >= taps,
-// n >= cout or c >= cin.  A wave's B fragment is one contiguous 512-byte
-// read.  W16 is [tap*cin + c][cout].
+// Wp in MFMA fragment order: fragment (s, j) is 64 lanes x 8 BF16, lane =
+// m + 16 q holding W[k][n = 16 j + m] for k = 32 s + 8 q .. + 7, where
+// k = tap * cinp + c; zero where tap >= taps, n >= cout or c >= cin.  A
+// wave's B fragment is one contiguous 1 KB read.  W16 is [tap*cin + c][cout].
 __global__ void pack_conv(const unsigned short *__restrict__ W16, unsigned short *__restrict__ Wp,
                           int taps, int cin, int cout, int cinp, int coutp) {
-    const int gs = cinp >= 16 ? cinp / 16 : 1, ng = cinp >= 16 ? taps : (taps + 1) / 2;
-    const int nt = coutp / 16;
-    const long long tot = (long long)ng * gs * nt * 256;
+    const int ns = (taps * cinp + 31) / 32, nt = coutp / 16;
+    const long long tot = (long long)ns * nt * 512;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
          i += (long long)gridDim.x * blockDim.x) {
-        const int e4 = (int)(i & 3), lane = (int)((i >> 2) & 63);
-        long long f = i >> 8;
-        const int j = (int)(f % nt);
-        f /= nt;
-        const int k = (int)(f % gs);
-        const int g = (int)(f / gs);
+        const int e = (int)(i & 7), lane = (int)((i >> 3) & 63);
+        const long long f = i >> 9;
+        const int j = (int)(f % nt), s = (int)(f / nt);
         const int m = lane & 15, q = lane >> 4, n = 16 * j + m;
-        const int tap = cinp >= 16 ? g : 2 * g + (q >> 1);
-        const int c = cinp >= 16 ? 16 * k + 4 * q + e4 : 4 * (q & 1) + e4;
+        const int k = 32 * s + 8 * q + e;
+        const int tap = k / cinp, c = k - tap * cinp;
         unsigned short v = 0;
         if (tap < taps && n < cout && c < cin) v = W16[((long long)tap * cin + c) * cout + n];
         Wp[i] = v;
@@ -680,7 +680,8 @@ void bf16_copy(hipStream_t st, const float *x, unsigned short *y, long long n) {
 int conv_cinp(int cin) { return (cin + 7) / 8 * 8; }
 int conv_coutp(int cout) { return cout <= 16 ? 16 : (cout <= 32 ? 32 : 64); }
 long long conv_packed_size(int ksz, int cin, int cout) {
-    return (long long)(ksz * ksz * ksz + 1) * conv_coutp(cout) * conv_cinp(cin);
+    // 32-wide k steps over taps x padded channels
+    return (long long)((ksz * ksz * ksz * conv_cinp(cin) + 31) / 32) * 32 * conv_coutp(cout);
 }
 
 void pack_conv_weights(hipStream_t st, const unsigned short *W16, unsigned short *Wp, int ksz,
