@@ -118,13 +118,14 @@ int st_fim_solve(st_ctx *ctx, const float *src_xyz, int nshot,
                  const st_fim_opts *opts, st_fim_stats *stats);
 
 /* As st_fim_solve (cold start, picks only), but the call allocates no
- * device memory: the block-layout slowness, travel times, schedule flags,
- * active lists, ordering keys, statistics, sources, receivers and picks all
+ * device memory: the block-layout slowness, travel times, their x-face
+ * copies, schedule flags, active lists, ordering keys, statistics, sources,
+ * receivers and picks all
  * live in one static device array of 64 MiB per GPU, held from library load
  * to process end.  No counterpart in the reference; it is for callers that
  * cannot allocate device memory (serving loops, harnesses that forbid
  * hipMalloc).  Limit: the slowness plus one shot's state must fit in the
- * 64 MiB (201x201x101 with 8^3 blocks: 37 MiB; shots are solved in passes
+ * 64 MiB (201x201x101 with 8^3 blocks: 41 MiB; shots are solved in passes
  * of as many as fit); larger models return ST_ERR_ARG.  opts: warm must be
  * 0, order 0 or 1; use_graph is ignored.  Every call on a GPU shares that
  * GPU's array: calls that overlap on one GPU (two threads, two contexts on
