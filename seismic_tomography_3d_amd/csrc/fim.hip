@@ -136,7 +136,7 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
     if (fm) {
         unsigned base = 0;
         if (lane == 0) base = atomicAdd(nxt_cnt, (unsigned)__popcll(fm));
-        base = __shfl(base, 0);
+        base = (unsigned)__builtin_amdgcn_readlane((int)base, 0);
         if (fresh) {
             const unsigned rank = (unsigned)__popcll(fm & ((1ull << lane) - 1ull));
             nxt[base + rank] = (shot << kShotShift) | target;
@@ -146,6 +146,39 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
 
 // Minimum over the lanes whose index differs from this lane's only in the
 // bits of mask (a butterfly over those bits).
+// Cross-lane minima in the VALU (DPP), without LDS permutes: a DPP move
+// reads a lane of the same 16-lane row (every lane active where these run).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_val(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+// minimum over each group of 8 lanes (quad swaps 1 and 2, then the half-row
+// mirror pairs the two quads)
+__device__ __forceinline__ float min8(float v) {
+    v = fminf(v, dpp_mov<0xB1>(v));    // quad_perm [1, 0, 3, 2]
+    v = fminf(v, dpp_mov<0x4E>(v));    // quad_perm [2, 3, 0, 1]
+    return fminf(v, dpp_mov<0x141>(v));  // row_half_mirror
+}
+// minimum over the whole wave, the same in every lane
+__device__ __forceinline__ float wave_min_all(float v) {
+    v = min8(v);
+    v = fminf(v, dpp_mov<0x140>(v));   // row_mirror: each row's 16 lanes
+    return fminf(fminf(lane_val(v, 0), lane_val(v, 16)), fminf(lane_val(v, 32), lane_val(v, 48)));
+}
+
+// maximum over the whole wave, the same in every lane
+__device__ __forceinline__ float wave_max_all(float v) {
+    v = fmaxf(v, dpp_mov<0xB1>(v));
+    v = fmaxf(v, dpp_mov<0x4E>(v));
+    v = fmaxf(v, dpp_mov<0x141>(v));
+    v = fmaxf(v, dpp_mov<0x140>(v));
+    return fmaxf(fmaxf(lane_val(v, 0), lane_val(v, 16)), fmaxf(lane_val(v, 32), lane_val(v, 48)));
+}
+
 template <int MASK>
 __device__ __forceinline__ float lane_min(float v) {
 #pragma unroll
@@ -247,7 +280,7 @@ __device__ __forceinline__ void for_items(const FimArgs &A, unsigned count, unsi
             if (fm) {
                 unsigned base = 0;
                 if (lane == 0) base = atomicAdd(nxt_cnt, (unsigned)__popcll(fm));
-                base = __shfl(base, 0);
+                base = (unsigned)__builtin_amdgcn_readlane((int)base, 0);
                 if (fresh) nxt[base + (unsigned)__popcll(fm & ((1ull << lane) - 1ull))] = item;
             }
             my_defer += (unsigned long long)__popcll(__ballot(in && !run));
@@ -512,11 +545,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             // octant: along each axis the front enters through the face
             // holding the earlier arrivals (diving waves reach shallow blocks
             // from below); on a tie, away from the source
-            auto wmin = [](float v) {
-#pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-                return v;
-            };
+            auto wmin = [](float v) { return wave_min_all(v); };
             const int fu = lane & 7, fw = lane >> 3;
             const float mxm = wmin(tile[tix(0, fu + 1, fw + 1)]);
             const float mxp = wmin(tile[tix(9, fu + 1, fw + 1)]);
@@ -667,13 +696,19 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         }
         float key = kKeyInf;
         if (A.tk) {
-            // x faces: min over ty (lanes tx + 8 ty), y faces: min over tx
-            const float mx = lane_min<56>(kx), my = lane_min<7>(ky);
-            const float mzm = lane_min<63>(fzm ? v[0] : kKeyInf);
-            const float mzp = lane_min<63>(fzp ? v[7] : kKeyInf);
-            const float mc = lane_min<63>(kc);
-            const float k1 = __shfl(mx, 0), k2 = __shfl(mx, 7), k3 = __shfl(my, 0),
-                        k4 = __shfl(my, 56);
+            // x faces: min over ty of the tx = 0 / 7 lanes (lanes tx + 8 ty;
+            // row_ror 8 pairs ty with ty ^ 1 inside a row, readlane the rest);
+            // y faces: min over tx of the ty = 0 / 7 rows of 8 lanes
+            const float mx = fminf(kx, dpp_mov<0x128>(kx));  // row_ror:8
+            const float my = min8(ky);
+            const float mzm = wave_min_all(fzm ? v[0] : kKeyInf);
+            const float mzp = wave_min_all(fzp ? v[7] : kKeyInf);
+            const float mc = wave_min_all(kc);
+            const float k1 = fminf(fminf(lane_val(mx, 0), lane_val(mx, 16)),
+                                   fminf(lane_val(mx, 32), lane_val(mx, 48)));
+            const float k2 = fminf(fminf(lane_val(mx, 7), lane_val(mx, 23)),
+                                   fminf(lane_val(mx, 39), lane_val(mx, 55)));
+            const float k3 = lane_val(my, 0), k4 = lane_val(my, 56);
             key = lane == 0 ? mc : lane == 1 ? k1 : lane == 2 ? k2 : lane == 3 ? k3
                 : lane == 4 ? k4 : lane == 5 ? mzm : mzp;
         }
@@ -825,10 +860,7 @@ struct CoarsePred {  // network output y on an nc^3 node cube spanning the model
     }
 };
 
-__device__ __forceinline__ float wave_min(float v) {
-    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off));
-    return v;
-}
+__device__ __forceinline__ float wave_min(float v) { return wave_min_all(v); }
 
 template <class Pred>
 __global__ __launch_bounds__(64) void warm_fill(float *__restrict__ T, float *__restrict__ keys,
@@ -1030,7 +1062,7 @@ __global__ void judge_residual(const float *__restrict__ T, const float *__restr
                                 fminf(at(x, y, z - 1), at(x, y, z + 1)), hs, hs * hs);
         r = fmaxf(g - v, 0.0f);
     }
-    for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
+    r = wave_max_all(r);
     if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&resmax[s], __float_as_uint(r));
     }
 }
@@ -1107,7 +1139,7 @@ __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
             r = fmaxf(r, godunov(a, bb, c, hs, hs * hs) - v[k + 1]);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) r = fmaxf(r, __shfl_xor(r, off));
+    r = wave_max_all(r);
     if (lane == 0 && r > 0.0f) atomicMax(&resmax[shot], __float_as_uint(r));
     wave_sync();  // the tile is restaged for the next item
     }
