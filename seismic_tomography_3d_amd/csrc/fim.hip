@@ -208,11 +208,13 @@ __device__ __forceinline__ void round_counts(const FimArgs &A, int lane, unsigne
                                              unsigned &count) {
     const int slot = A.round % 3;
     unsigned v = lane < kSeg ? A.cnt[(slot * kSeg + lane) * kCntStride] : 0u;
-#pragma unroll
-    for (int o = 1; o < kSeg; o <<= 1) {
-        const unsigned t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
+    // inclusive scan inside the first 16-lane row by DPP row shifts (a lane
+    // with no source lane takes 0)
+    static_assert(kSeg == 16, "one DPP row of segments");
+    v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
     incl = v;
     count = (unsigned)__builtin_amdgcn_readlane((int)v, kSeg - 1);
 }

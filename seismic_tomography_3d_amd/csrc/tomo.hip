@@ -605,6 +605,18 @@ __global__ void col_gather(const long long *__restrict__ rowptr, int nrows,
     tval[k] = vals[e];
 }
 
+// Sum over each group of 16 lanes, the same in all of them: the butterfly of
+// lane ^ 8, ^ 4, ^ 2, ^ 1, taken by DPP row rotations (after the step with
+// 8 the values repeat every 8 lanes, so rotating by 4 pairs the same lanes
+// as ^ 4, and so on; x + y = y + x keeps every lane's bits).
+__device__ __forceinline__ float row16_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x128, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x124, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x122, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x121, 0xF, 0xF, false));
+    return v;
+}
+
 // acc += val[k] * x[idx[k]] for k = k0, k0 + S, ... below e, in that order,
 // with four entries' loads (index, value, then the gathered x) in flight at
 // a time instead of one: the same sequence of sums, fewer round trips.
@@ -638,7 +650,7 @@ __global__ void spmv_csr(const long long *__restrict__ rowptr, const int *__rest
         const long long b = rowptr[row], e = rowptr[row + 1];
         acc = strided_dot<16>(val, idx, x, b + l, e, acc);
     }
-    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    acc = row16_sum(acc);
     if (row < nrows && l == 0) y[row] = acc;
 }
 
@@ -684,7 +696,7 @@ __global__ __launch_bounds__(256) void spmv_split(const long long *__restrict__ 
             mine = e - b <= kLongCol;
             if (mine) acc = strided_dot<16>(val, idx, x, b + l, e, acc);
         }
-        for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+        acc = row16_sum(acc);
         if (mine && l == 0) y[row] = acc;
         return;
     }
