@@ -636,12 +636,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         float v[8];
         #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = vz[k + 1];
-        if constexpr (HYP) {
-            // the visit's starting values are still in memory (this wave
-            // writes the block back only below): reloaded instead of kept
-            #pragma unroll
-            for (int k = 0; k < 8; k++) v0[k] = Tsh[base + lane + 64 * k];
-        }
+
 
         // face activation as in the 4^3 kernel: a changed face cell must be
         // below the neighbour's adjacent cell (tile halo / z halo registers)
