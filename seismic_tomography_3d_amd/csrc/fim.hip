@@ -1143,26 +1143,38 @@ __global__ __launch_bounds__(64 * kWaves) void judge_blocks8(
 }
 }  // namespace st
 
-// Sums the per-wave statistics slots into out[0..2] (one workgroup).
-__global__ __launch_bounds__(256) void sum_wave_stats(const unsigned long long *__restrict__ w,
-                                                      int nw, unsigned long long *__restrict__ out) {
-    __shared__ unsigned long long part[3][256];
+// Sums the per-wave statistics slots into out[0..2] (one workgroup of 1,024
+// threads, four slots per thread in flight: the sum is latency-bound).
+__global__ __launch_bounds__(1024) void sum_wave_stats(const unsigned long long *__restrict__ w,
+                                                       int nw, unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long part[3][1024];
     unsigned long long a = 0, b = 0, c = 0;
-    for (int i = threadIdx.x; i < nw; i += 256) {
-        a += w[3 * i];
-        b += w[3 * i + 1];
-        c += w[3 * i + 2];
+    const int t = threadIdx.x;
+    for (int i = t; i < nw; i += 4 * 1024) {
+        unsigned long long v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = i + u * 1024;
+#pragma unroll
+            for (int k = 0; k < 3; k++) v[u][k] = j < nw ? w[3LL * j + k] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a += v[u][0];
+            b += v[u][1];
+            c += v[u][2];
+        }
     }
-    part[0][threadIdx.x] = a;
-    part[1][threadIdx.x] = b;
-    part[2][threadIdx.x] = c;
+    part[0][t] = a;
+    part[1][t] = b;
+    part[2][t] = c;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (threadIdx.x < h)
-            for (int k = 0; k < 3; k++) part[k][threadIdx.x] += part[k][threadIdx.x + h];
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h)
+            for (int k = 0; k < 3; k++) part[k][t] += part[k][t + h];
         __syncthreads();
     }
-    if (threadIdx.x < 3) out[threadIdx.x] = part[threadIdx.x][0];
+    if (t < 3) out[t] = part[t][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1287,7 +1299,11 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     A.tk = tk;
     A.xf = xf;
 
-    const int nwaves = fim_waves(o);
+    // no more waves than an eighth of the batch's blocks (at least 1,024): a
+    // small batch, such as the cold re-solve of rejected warm starts, keeps
+    // its rounds from launching waves that find no item
+    const long long wcap = std::max<long long>(1024, (long long)nshot * g.nblk / 8);
+    const int nwaves = (int)std::min<long long>(fim_waves(o), (wcap + kWaves - 1) / kWaves * kWaves);
     const int grid = nwaves / kWaves;
     if (stats) {
         ST_HIP(hipMemsetAsync(bf.wstats, 0, sizeof(unsigned long long) * 3 * nwaves, ctx->stream));
@@ -1379,7 +1395,7 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     }
     ST_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     if (stats) {
-        hipLaunchKernelGGL(sum_wave_stats, dim3(1), dim3(256), 0, ctx->stream, bf.wstats, nwaves,
+        hipLaunchKernelGGL(sum_wave_stats, dim3(1), dim3(1024), 0, ctx->stream, bf.wstats, nwaves,
                            dstats);
         ST_HIP(hipGetLastError());
         ST_HIP(hipMemcpyAsync(hcnt + 4, cnt + kRoundsIdx, sizeof(unsigned), hipMemcpyDeviceToHost,
