@@ -585,23 +585,29 @@ __global__ void iota_int(int *__restrict__ v, long long n) {
     if (e < n) v[e] = (int)e;
 }
 
-// A^T gather: entry k of the column-sorted order is A's entry e = idx[k];
-// its row is found by binary search of rowptr.  The sort is stable and A's
-// entries are in row order, so rows ascend inside each column -- the
-// summation order of the reference's sequential COO loop
-// (least_squares.cpp:440, :479), which keeps SpMV^T deterministic.
-__global__ void col_gather(const long long *__restrict__ rowptr, int nrows,
-                           const float *__restrict__ vals, const int *__restrict__ idx,
-                           long long nnz, int *__restrict__ trow, float *__restrict__ tval) {
+// Row of every entry of A: 16 lanes per row write its entries' row index
+// (coalesced runs of the row's span).
+__global__ void entry_rows(const long long *__restrict__ rowptr, int nrows, int *__restrict__ rowid) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long row = gid >> 4;
+    if (row >= nrows) return;
+    const int l = (int)(gid & 15);
+    const long long e = rowptr[row + 1];
+    for (long long k = rowptr[row] + l; k < e; k += 16) rowid[k] = (int)row;
+}
+
+// A^T gather: entry k of the column-sorted order is A's entry e = idx[k], of
+// row rowid[e].  The sort is stable and A's entries are in row order, so rows
+// ascend inside each column -- the summation order of the reference's
+// sequential COO loop (least_squares.cpp:440, :479), which keeps SpMV^T
+// deterministic.
+__global__ void col_gather(const int *__restrict__ rowid, const float *__restrict__ vals,
+                           const int *__restrict__ idx, long long nnz, int *__restrict__ trow,
+                           float *__restrict__ tval) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nnz) return;
     const long long e = idx[k];
-    int lo = 0, hi = nrows - 1;
-    while (lo < hi) {  // last row with rowptr[row] <= e
-        const int mid = (lo + hi + 1) >> 1;
-        if (rowptr[mid] <= e) lo = mid; else hi = mid - 1;
-    }
-    trow[k] = lo;
+    trow[k] = rowid[e];
     tval[k] = vals[e];
 }
 
@@ -873,9 +879,12 @@ void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda) {
                                      ctx->stream));
     hipLaunchKernelGGL(col_bounds, dim3(nblk256(annz + 1)), dim3(256), 0, ctx->stream,
                        t.t_keys.as<unsigned>(), annz, n_model, t.t_colptr.as<long long>());
+    // the radix sort's input indices (t_idx0) are free again: each entry's row
+    hipLaunchKernelGGL(entry_rows, dim3(nblk256(N * 16)), dim3(256), 0, ctx->stream,
+                       t.rowptr.as<long long>(), (int)N, t.t_idx0.as<int>());
     hipLaunchKernelGGL(col_gather, dim3(nblk256(annz)), dim3(256), 0, ctx->stream,
-                       t.rowptr.as<long long>(), (int)N, t.vals.as<float>(), t.t_idx1.as<int>(),
-                       annz, t.t_rows.as<int>(), t.t_vals.as<float>());
+                       t.t_idx0.as<int>(), t.vals.as<float>(), t.t_idx1.as<int>(), annz,
+                       t.t_rows.as<int>(), t.t_vals.as<float>());
     // A^T's long columns: t_cursor[0] counts them, t_cursor[1..] lists them
     hipLaunchKernelGGL(zero_vec, dim3(1), dim3(64), 0, ctx->stream,
                        reinterpret_cast<float *>(t.t_cursor.ptr), 1LL);
