@@ -54,6 +54,53 @@ def test_single_shot_case_vs_fp64(ctx):
     assert err < TOL and perr < TOL
 
 
+def test_c1_grid_against_analytic_and_the_other_solvers(ctx):
+    """The 101x101x51 grid of the single-shot case with its anomaly off
+    (v = 1500 + 2 z m/s), where the traveltime is known in closed form,
+    T = arccosh(1 + g^2 R^2 / (2 v_s v)) / g: the block FIM, the device
+    Accurate_FSM and the device Podvin solver against it and against each
+    other.  All three schemes overestimate a point source's times; measured
+    with the CPU oracles (max / RMS over the 20 x 20 receivers, ms): FIM
+    30.0 / 21.7, Podvin 17.1 / 12.7, Accurate_FSM 12.3 / 8.8.  The FIM is
+    also held to its FP64 fixed point (1e-4 s), the two others to their CPU
+    code bit for bit."""
+    c = synth.single_shot_case()
+    nx, ny, nz, h = c["nx"], c["ny"], c["nz"], c["h"]
+    vel = synth.gradient_model(nx, ny, nz, h, amp=0.0)
+    ctx.set_model(1.0 / vel, h)
+    src = c["src"]
+    rcv = np.stack([c["rx"], c["ry"], c["rz"]], axis=1).astype(np.float32)
+    g = 2500.0 / ((nz - 1) * h)
+    R2 = ((rcv - src[0]) ** 2).sum(axis=1).astype(np.float64)
+    Ta = np.arccosh(1 + g * g * R2 / (2 * 1500.0 * (1500.0 + g * rcv[:, 2]))) / g
+
+    pf, _, _ = ctx.fim_solve(src, rcv, block=8, batch=1)
+    ps, _, _ = ctx.fsm_solve(src, rcv)
+    pp, _, _ = ctx.podvin_solve(src, rcv)
+    errs = {}
+    for name, p in (("fim", pf[0]), ("fsm", ps[0]), ("podvin", pp[0])):
+        e = p.astype(np.float64) - Ta
+        errs[name] = (e.min(), np.abs(e).max(), np.sqrt((e * e).mean()))
+        print(f"{name}: vs analytic min {e.min() * 1e3:.2f} max {np.abs(e).max() * 1e3:.2f} "
+              f"rms {np.sqrt((e * e).mean()) * 1e3:.2f} ms")
+    # above the exact times (first-order point-source error), at the ms scale
+    for name, (lo, mx, rms) in errs.items():
+        assert lo > -1e-4, name
+    assert errs["fim"][1] < 0.035 and errs["fim"][2] < 0.025
+    assert errs["podvin"][1] < 0.020 and errs["podvin"][2] < 0.015
+    assert errs["fsm"][1] < 0.015 and errs["fsm"][2] < 0.011
+    # the schemes' accuracy order: Accurate_FSM, then Podvin, then the FIM
+    assert errs["fsm"][2] < errs["podvin"][2] < errs["fim"][2]
+    # against each other: the FIM's times above the two others' by less than 25 ms
+    assert (pf[0] - ps[0]).min() > -1e-4 and (pf[0] - ps[0]).max() < 0.025
+    assert (pf[0] - pp[0]).min() > -1e-3 and (pf[0] - pp[0]).max() < 0.025
+    # each against its own reference computation
+    T64 = orc.eikonal_fp64(vel, h, src[0])
+    assert np.abs(pf[0] - orc.picks(T64.astype(np.float32), (h, h, h), rcv)).max() < TOL
+    Tf = orc.fsm(vel, (h, h, h), tuple(float(v) for v in src[0]))
+    assert np.array_equal(ps[0], orc.picks(Tf, (h, h, h), rcv).astype(np.float32))
+
+
 def test_batch_matches_single(ctx):
     vel, h = _case(seed=3)
     ctx.set_model(1.0 / vel, h)
