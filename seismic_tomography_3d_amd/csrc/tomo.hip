@@ -51,6 +51,11 @@ struct TomoState {
     long long annz = 0;
     DevBuf t_colptr, t_rows, t_vals, t_cursor;
     int n_long = 0;                // A^T columns reduced by a whole workgroup
+    // A^T in row slices (large systems): per column kSlices + 1 entry bounds,
+    // kSlices partial sums per column, the long pieces per slice
+    DevBuf t_bounds, t_part, t_slong;
+    bool sliced = false;
+    int n_slong = 0;               // most long pieces in one slice
     DevBuf t_keys, t_idx0, t_idx1, t_sort;  // column-sort staging (radix sort pairs)
     DevBuf b, s, q, r, p, x, dots, scal;
     bool have_x = false;
@@ -71,7 +76,7 @@ void tomo_release(st_ctx *ctx) {
     TomoState &t = *static_cast<TomoState *>(ctx->tomo);
     for (DevBuf *b : {&t.rowptr, &t.cols, &t.vals, &t.runs, &t.nruns, &t.fdist, &t.rowlen,
                       &t.scanbuf, &t.err, &t.illum, &t.t_colptr, &t.t_rows, &t.t_vals,
-                      &t.t_cursor, &t.t_keys, &t.t_idx0, &t.t_idx1, &t.t_sort, &t.b, &t.s, &t.q, &t.r, &t.p, &t.x, &t.dots, &t.scal, &t.dm,
+                      &t.t_cursor, &t.t_bounds, &t.t_part, &t.t_slong, &t.t_keys, &t.t_idx0, &t.t_idx1, &t.t_sort, &t.b, &t.s, &t.q, &t.r, &t.p, &t.x, &t.dots, &t.scal, &t.dm,
                       &t.upd, &t.kern, &t.gkern})
         b->release();
     if (t.gexec) (void)hipGraphExecDestroy(t.gexec);
@@ -399,12 +404,18 @@ __global__ void ray_trace(RayArgs a, L lay, int per) {
 
 // CSR values: first (smallest) cell gets rayStep + n additions, later cells
 // rayStep + (n-1) additions (sequential FP32 sums), the source cell finalDist.
-__global__ void ray_fill(const int2 *__restrict__ runs, const int *__restrict__ nruns,
-                         const float *__restrict__ fdist, const long long *__restrict__ rowptr,
-                         int *__restrict__ cols, float *__restrict__ vals, long long nrays,
-                         int max_runs, const float *__restrict__ src, int nr, float dz, float dxt,
-                         float dyt, float dzt, int nxt, int nzt) {
-    const long long ray = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per ray, one lane per run: the runs are read and the row written
+// as contiguous 64-entry spans; each lane's sum stays sequential.
+constexpr int kFillRays = 4;  // rays (waves) per 256-thread workgroup
+
+__global__ void __launch_bounds__(64 * kFillRays)
+ray_fill(const int2 *__restrict__ runs, const int *__restrict__ nruns,
+         const float *__restrict__ fdist, const long long *__restrict__ rowptr,
+         int *__restrict__ cols, float *__restrict__ vals, long long nrays, int max_runs,
+         const float *__restrict__ src, int nr, float dz, float dxt, float dyt, float dzt,
+         int nxt, int nzt) {
+    const long long ray = (long long)blockIdx.x * kFillRays + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (ray >= nrays) return;
     const int s = (int)(ray / nr);
     const float sx = src[3 * s], sy = src[3 * s + 1], sz = src[3 * s + 2];
@@ -413,13 +424,14 @@ __global__ void ray_fill(const int2 *__restrict__ runs, const int *__restrict__ 
     const int2 *my = runs + ray * max_runs;
     const long long o = rowptr[ray];
     const int m = nruns[ray];
-    for (int u = 0; u < m; u++) {
-        const int n = my[u].y;
+    const float fd = fdist[ray];
+    for (int u = lane; u < m; u += 64) {
+        const int2 r = my[u];
         float d = rayStep;
-        const int adds = (u == 0) ? n : n - 1;
+        const int adds = (u == 0) ? r.y : r.y - 1;
         for (int q = 0; q < adds; q++) d += rayStep;
-        cols[o + u] = my[u].x;
-        vals[o + u] = (my[u].x == sId) ? fdist[ray] : d;
+        cols[o + u] = r.x;
+        vals[o + u] = (r.x == sId) ? fd : d;
     }
 }
 
@@ -544,7 +556,9 @@ static void trace_append(st_ctx *ctx, const float *T, long long vol, L lay, cons
                           ctx->stream));
     grow_keep(t.cols, sizeof(int) * new_nnz, sizeof(int) * t.nnz, ctx->stream);
     grow_keep(t.vals, sizeof(float) * new_nnz, sizeof(float) * t.nnz, ctx->stream);
-    hipLaunchKernelGGL(ray_fill, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, ctx->stream,
+    ST_REQUIRE(nrays < (1LL << 26), ST_ERR_CAPACITY, "too many rays in one ray-tracing batch");
+    hipLaunchKernelGGL(ray_fill, dim3((unsigned)((nrays + kFillRays - 1) / kFillRays)),
+                       dim3(64 * kFillRays), 0, ctx->stream,
                        t.runs.as<int2>(), t.nruns.as<int>(), t.fdist.as<float>(), rp,
                        t.cols.as<int>(), t.vals.as<float>(), nrays, t.max_runs, dsrc, nr, ctx->dz,
                        t.dxt, t.dyt, t.dzt, t.nxt, t.nzt);
@@ -720,6 +734,102 @@ __global__ __launch_bounds__(256) void spmv_split(const long long *__restrict__ 
     if (threadIdx.x == 0) y[row] = sh[0];
 }
 
+// A^T products of large systems.  The input vector of A^T x has a value per
+// ray; beyond an XCD's 4 MB L2 (a million rays) every gathered value costs a
+// cache line from farther out.  Each column's entries (rows ascending) are
+// cut into kSlices ranges of rows; the workgroups of slice s are those with
+// w % kSlices == s, which the dispatcher deals to XCD s, so an XCD gathers
+// from one eighth of the vector only.  Slices give partial sums per column,
+// added in a fixed order afterwards: results repeat run to run.
+constexpr int kSlices = 8;
+constexpr long long kSliceRows = 1LL << 20;  // systems with at least this many rows
+
+// bounds[c * (kSlices + 1) + s]: first entry of column c whose row is at
+// least s * rps (s = 0 and kSlices: the column's ends); pieces longer than
+// kLongCol entries are listed per slice for a whole workgroup each.
+__global__ void slice_bounds(const long long *__restrict__ colptr, const int *__restrict__ rows,
+                             int ncols, long long rps, long long *__restrict__ bounds,
+                             int *__restrict__ slong_cnt, int *__restrict__ slong_list) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    const long long e = colptr[c + 1];
+    long long prev = colptr[c];
+    bounds[(long long)c * (kSlices + 1)] = prev;
+    for (int s = 1; s <= kSlices; s++) {
+        long long pos = e;
+        if (s < kSlices) {
+            const long long target = s * rps;
+            long long lo = prev, hi = e;
+            while (lo < hi) {
+                const long long mid = (lo + hi) >> 1;
+                if (rows[mid] < target) lo = mid + 1; else hi = mid;
+            }
+            pos = lo;
+        }
+        bounds[(long long)c * (kSlices + 1) + s] = pos;
+        if (pos - prev > kLongCol)
+            slong_list[(long long)(s - 1) * ncols + atomicAdd(&slong_cnt[s - 1], 1)] = c;
+        prev = pos;
+    }
+}
+
+// part[s * ncols + c] = the slice-s piece of column c times x: workgroups
+// j < nshort give 16 lanes to each of 16 columns' pieces of at most
+// kLongCol entries, the rest one workgroup to each listed long piece.
+__global__ __launch_bounds__(256) void spmv_t_sliced(const long long *__restrict__ bounds,
+                                                     const int *__restrict__ idx,
+                                                     const float *__restrict__ val,
+                                                     const float *__restrict__ x,
+                                                     float *__restrict__ part, int ncols, int nshort,
+                                                     const int *__restrict__ slong_cnt,
+                                                     const int *__restrict__ slong_list) {
+    const int s = (int)(blockIdx.x % kSlices);
+    const int j = (int)(blockIdx.x / kSlices);
+    if (j < nshort) {
+        const int c = (j * 256 + (int)threadIdx.x) >> 4;
+        const int l = threadIdx.x & 15;
+        float acc = 0.f;
+        bool mine = false;
+        if (c < ncols) {
+            const long long b = bounds[(long long)c * (kSlices + 1) + s];
+            const long long e = bounds[(long long)c * (kSlices + 1) + s + 1];
+            mine = e - b <= kLongCol;
+            if (mine) acc = strided_dot<16>(val, idx, x, b + l, e, acc);
+        }
+        acc = row16_sum(acc);
+        if (mine && l == 0) part[(long long)s * ncols + c] = acc;
+        return;
+    }
+    if (j - nshort >= slong_cnt[s]) return;  // the whole workgroup
+    __shared__ float sh[256];
+    const int c = slong_list[(long long)s * ncols + (j - nshort)];
+    const long long b = bounds[(long long)c * (kSlices + 1) + s];
+    const long long e = bounds[(long long)c * (kSlices + 1) + s + 1];
+    float acc = 0.f;
+    acc = strided_dot<256>(val, idx, x, b + threadIdx.x, e, acc);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long long)s * ncols + c] = sh[0];
+}
+
+// y[c] = the kSlices partial sums of column c, pairwise in a fixed order
+__global__ void sum_slices(const float *__restrict__ part, int ncols, float *__restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    float v[kSlices];
+#pragma unroll
+    for (int s = 0; s < kSlices; s++) v[s] = part[(long long)s * ncols + c];
+#pragma unroll
+    for (int w = 1; w < kSlices; w <<= 1)
+#pragma unroll
+        for (int s = 0; s < kSlices; s += 2 * w) v[s] += v[s + w];
+    y[c] = v[0];
+}
+
 // FP64 dot products: per-block partials, then one ordered pass
 __global__ void dot_partial(const float *__restrict__ a, const float *__restrict__ b, long long n,
                             double *__restrict__ part) {
@@ -891,11 +1001,30 @@ void tomo_cgls_setup(st_ctx *ctx, int n_model, int order, float lambda) {
     hipLaunchKernelGGL(long_rows, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream,
                        t.t_colptr.as<long long>(), n_model, t.t_cursor.as<int>(),
                        t.t_cursor.as<int>() + 1);
+    t.sliced = N >= kSliceRows;
+    if (t.sliced) {
+        t.t_bounds.reserve(sizeof(long long) * (kSlices + 1) * (size_t)n_model);
+        t.t_part.reserve(sizeof(float) * kSlices * (size_t)n_model);
+        t.t_slong.reserve(sizeof(int) * kSlices * ((size_t)n_model + 1));
+        int *slong_cnt = t.t_slong.as<int>();
+        hipLaunchKernelGGL(zero_vec, dim3(1), dim3(64), 0, ctx->stream,
+                           reinterpret_cast<float *>(slong_cnt), (long long)kSlices);
+        hipLaunchKernelGGL(slice_bounds, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream,
+                           t.t_colptr.as<long long>(), t.t_rows.as<int>(), n_model,
+                           (N + kSlices - 1) / kSlices, t.t_bounds.as<long long>(), slong_cnt,
+                           slong_cnt + kSlices);
+    }
     ST_HIP(hipGetLastError());
     int *hcount = static_cast<int *>(ctx->host_pinned) + 64;
     ST_HIP(hipMemcpyAsync(hcount, t.t_cursor.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (t.sliced)
+        ST_HIP(hipMemcpyAsync(hcount + 1, t.t_slong.ptr, sizeof(int) * kSlices,
+                              hipMemcpyDeviceToHost, ctx->stream));
     ST_HIP(hipStreamSynchronize(ctx->stream));
     t.n_long = *hcount;
+    t.n_slong = 0;
+    if (t.sliced)
+        for (int k = 0; k < kSlices; k++) t.n_slong = std::max(t.n_slong, hcount[1 + k]);
 
     t.N = (int)N; t.M = n_model; t.annz = annz; t.n_model = n_model; t.order = order;
     t.s.reserve(sizeof(float) * N);
@@ -928,6 +1057,16 @@ void tomo_cgls_iterate(st_ctx *ctx, int iters) {
     };
     auto spmvt = [&](const float *in, float *out) {  // out = A^T in
         const int nshort = (int)(((long long)n_model * 16 + 255) / 256);
+        if (t.sliced) {
+            const int *slong = t.t_slong.as<int>();
+            hipLaunchKernelGGL(spmv_t_sliced, dim3((unsigned)(kSlices * (nshort + t.n_slong))),
+                               dim3(256), 0, ctx->stream, t.t_bounds.as<long long>(),
+                               t.t_rows.as<int>(), t.t_vals.as<float>(), in, t.t_part.as<float>(),
+                               n_model, nshort, slong, slong + kSlices);
+            hipLaunchKernelGGL(sum_slices, dim3(nblk256(n_model)), dim3(256), 0, ctx->stream,
+                               t.t_part.as<float>(), n_model, out);
+            return;
+        }
         hipLaunchKernelGGL(spmv_split, dim3((unsigned)(nshort + t.n_long)), dim3(256), 0,
                            ctx->stream, cp, t.t_rows.as<int>(), t.t_vals.as<float>(), in, out,
                            n_model, nshort, t.t_cursor.as<int>() + 1);
@@ -1206,6 +1345,8 @@ void tomo_solve_update(st_ctx *ctx, int iters, int smooth, int samples, float st
         (long long)t.p.ptr, (long long)t.x.ptr, (long long)t.dots.ptr, (long long)t.scal.ptr,
         (long long)t.rowptr.ptr, (long long)t.cols.ptr, (long long)t.vals.ptr,
         (long long)t.t_colptr.ptr, (long long)t.t_rows.ptr, (long long)t.t_vals.ptr,
+        (long long)t.t_cursor.ptr, t.n_long, (long long)t.t_bounds.ptr, (long long)t.t_part.ptr,
+        (long long)t.t_slong.ptr, t.sliced ? 1LL + t.n_slong : 0LL,
         (long long)t.dm.ptr, (long long)t.upd.ptr, (long long)t.gkern.ptr,
         (long long)ctx->slow_zxy.ptr, t.N, t.n_model, t.annz, iters, smooth, samples, ctx->nx,
         ctx->ny, ctx->nz, t.nxt, t.nyt, t.nzt};

@@ -113,6 +113,48 @@ def test_cgls_equals_lstsq(ctx):
     assert np.abs(x - ref).max() < 1e-3 * np.abs(ref).max()
 
 
+def test_cgls_sliced_transpose_large_system(ctx):
+    """A system of more than 2^20 rows: A^T products run in row slices, one
+    per XCD, with a column in every row (pieces reduced by a workgroup) and
+    one in every third row.  The first CGLS step is alpha A^T b; ten steps
+    follow the FP64 recurrence; two solves give the same bits."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(11)
+    N, M, k = (1 << 20) + 4321, 3000, 6
+    start = rng.integers(2, M - 7 * k, size=N)
+    rest = start[:, None] + 7 * np.arange(k)[None, :]
+    every3 = (np.arange(N) % 3 == 0)
+    counts = 1 + every3.astype(np.int64) + k
+    rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    cols = np.empty(rp[-1], np.int32)
+    cols[rp[:-1]] = 0
+    cols[rp[:-1][every3] + 1] = 1
+    body = rp[:-1] + 1 + every3
+    for j in range(k):
+        cols[body + j] = rest[:, j]
+    # the dense columns scaled to norms like the others': unscaled, they
+    # leave FP32 CGLS itself (vectors rounded each step) a few % off the FP64
+    # recurrence after ten steps
+    w = np.ones(M, np.float32)
+    w[0], w[1] = 0.04, 0.07
+    vals = (rng.normal(size=rp[-1]).astype(np.float32) * w[cols]).astype(np.float32)
+    b = rng.normal(size=N).astype(np.float32)
+    A = sp.csr_matrix((vals.astype(np.float64), cols, rp), shape=(N, M))
+    g = A.T @ b.astype(np.float64)
+    x1 = ctx.cgls_csr(rp, cols, vals, M, b, 1).astype(np.float64)
+    alpha = x1 @ g / (g @ g)
+    err1 = np.abs(x1 - alpha * g).max() / np.abs(alpha * g).max()
+    print(f"{N} rows, {rp[-1]} non-zeros: first step vs alpha A^T b {err1:.2e}")
+    assert alpha > 0 and err1 < 1e-5
+    x = ctx.cgls_csr(rp, cols, vals, M, b, 10)
+    x64 = np.zeros(M, np.float64)
+    orc.oracle().or_cgls_f64(P(rp), P(cols), P(vals), N, M, P(b), 10, P(x64))
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    print(f"ten steps vs the FP64 recurrence: {err:.2e}")
+    assert err < 1e-4
+    assert np.array_equal(x, ctx.cgls_csr(rp, cols, vals, M, b, 10))
+
+
 def test_optimize_vs_fp64_recurrence(ctx):
     vel, h, src, rcv = _small_tomo()
     ctx.set_model(1.0 / vel, h)
