@@ -70,6 +70,18 @@ struct FimArgs {
     // neighbour's x halo is 64 contiguous floats instead of one float in each
     // 32-byte row of the block
     float *xf;
+    // 8^3 cold starts without the volume fills: the first wave to schedule
+    // a block claims it (flag bit 2), fills its cells and x faces with kInf
+    // and stamps the flag word (bits 3 and up) with epoch + 2 of its launch
+    // (1: the source blocks, filled before the first launch).  A block is
+    // first visited in a later launch than its claim, so every visited block
+    // is filled; a visit trusts a neighbour stamped by an earlier launch
+    // (its fill and writes are visible) and reads any other neighbour as
+    // unreached (kInf) -- unclaimed, or claimed in this launch and so not
+    // yet visited: what a cold fill would hold.  Blocks never scheduled are
+    // filled after the solve.
+    int lazy;
+    int epoch;                 // launch index in the call (lazy only)
 };
 
 constexpr unsigned kInfBits = 0x7F800000u;
@@ -115,10 +127,13 @@ __device__ __forceinline__ float godunov(float a, float b, float c, float hs, fl
 __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsigned *fl,
                                          unsigned target, unsigned bit_nxt, unsigned *nxt,
                                          unsigned *nxt_cnt, int lane, float key) {
-    bool fresh = false;
+    bool fresh = false, claim = false;
     if (target != 0xFFFFFFFFu) {
-        const unsigned old = atomicOr(&fl[target], bit_nxt);
+        const unsigned old = atomicOr(&fl[target], bit_nxt | (A.lazy ? 4u : 0u));
         fresh = !(old & bit_nxt);
+        claim = A.lazy && !(old & 4u);
+        // the claim's stamp (one writer, so an OR)
+        if (claim) atomicOr(&fl[target], ((unsigned)A.epoch + 2u) << 3);
         if (A.tk) {
             // every scheduler lowers the next list's key (the item itself is
             // appended once, by the first); with predictions too, the block
@@ -141,6 +156,19 @@ __device__ __forceinline__ void schedule(const FimArgs &A, unsigned shot, unsign
             const unsigned rank = (unsigned)__popcll(fm & ((1ull << lane) - 1ull));
             nxt[base + rank] = (shot << kShotShift) | target;
         }
+    }
+    // claimed blocks' cold fill, by the whole wave: 512 cells, 128 face values
+    unsigned long long cm = __ballot(claim);
+    while (cm) {
+        const int j = __builtin_ctzll(cm);
+        cm &= cm - 1ull;
+        const long long tb = (long long)(unsigned)__builtin_amdgcn_readlane((int)target, j);
+        float4 *p = reinterpret_cast<float4 *>(A.T + (long long)shot * A.vol + tb * 512);
+        p[lane] = make_float4(kInf, kInf, kInf, kInf);
+        p[lane + 64] = make_float4(kInf, kInf, kInf, kInf);
+        if (A.xf)
+            reinterpret_cast<float2 *>(A.xf + ((long long)shot * A.nblk + tb) * 128)[lane] =
+                make_float2(kInf, kInf);
     }
 }
 
@@ -498,8 +526,25 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         const unsigned shot = item >> kShotShift, b = item & kBlkMask;
         float *Tsh = A.T + (long long)shot * A.vol;
         unsigned *fl = A.flags + (long long)shot * A.nblk;
-        if (lane == 0) atomicAnd(&fl[b], ~bit_cur);
         const int bx = b % A.nbx, rest = b / A.nbx, by = rest % A.nby, bz = rest / A.nby;
+        if (lane == 0) atomicAnd(&fl[b], ~bit_cur);
+        // lazy starts: lanes 1..6 load the flag words of the -x, +x, -y, +y,
+        // -z, +z neighbours (in flight with the block's own values); okm bit
+        // i: neighbour i's values are trusted (bit 0, the block itself, always)
+        unsigned fw = 0u, okm = 0x7Fu;
+        if (A.lazy) {
+            long long nb = -1;
+            switch (lane) {
+            case 1: if (bx > 0) nb = (long long)b - 1; break;
+            case 2: if (bx < A.nbx - 1) nb = (long long)b + 1; break;
+            case 3: if (by > 0) nb = (long long)b - A.nbx; break;
+            case 4: if (by < A.nby - 1) nb = (long long)b + A.nbx; break;
+            case 5: if (bz > 0) nb = (long long)b - nbxy; break;
+            case 6: if (bz < A.nbz - 1) nb = (long long)b + nbxy; break;
+            default: break;
+            }
+            if (nb >= 0) fw = fl[nb];
+        }
         const long long base = (long long)b * 512;
         const bool vxy = (bx * 8 + tx < A.n4x) && (by * 8 + ty < A.n4y);
 
@@ -509,34 +554,38 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         for (int k = 0; k < 8; k++) {
             vz[k + 1] = Tsh[base + lane + 64 * k];
             hs[k] = A.h * A.S[base + lane + 64 * k];
+        }
+        // the halos are loaded whenever the neighbour exists, in flight with
+        // the flag words, and kept if trusted: a neighbour's trust cannot
+        // change within a launch, so values read from an untrusted block are
+        // never used
+        const float zlo = (bz > 0) ? Tsh[base - 512LL * nbxy + lane + 64 * 7] : kInf;
+        const float zhi = (bz < A.nbz - 1) ? Tsh[base + 512LL * nbxy + lane] : kInf;
+        // x / y face halos: face f = 0..3 (-x, +x, -y, +y), lane = u + 8 w
+        // (u = in-face coordinate, w = z)
+        const int u = lane & 7, w = lane >> 3;
+        float hv[4];
+        hv[0] = bx > 0 ? (A.xf ? A.xf[((long long)shot * A.nblk + b - 1) * 128 + 64 + lane]
+                               : Tsh[base - 512 + 7 + 8 * u + 64 * w]) : kInf;
+        hv[1] = bx < A.nbx - 1 ? (A.xf ? A.xf[((long long)shot * A.nblk + b + 1) * 128 + lane]
+                                       : Tsh[base + 512 + 8 * u + 64 * w]) : kInf;
+        hv[2] = by > 0 ? Tsh[base - 512LL * A.nbx + u + 56 + 64 * w] : kInf;
+        hv[3] = by < A.nby - 1 ? Tsh[base + 512LL * A.nbx + u + 64 * w] : kInf;
+        if (A.lazy) {
+            const unsigned st = fw >> 3;
+            okm = (unsigned)__ballot(st != 0u && st <= (unsigned)A.epoch + 1u) | 1u;
+        }
+        #pragma unroll
+        for (int k = 0; k < 8; k++) {
             v0[k] = vz[k + 1];
             tile[tcol + kTZ * (k + 1)] = vz[k + 1];
         }
-        vz[0] = (bz > 0) ? Tsh[base - 512LL * nbxy + lane + 64 * 7] : kInf;
-        vz[9] = (bz < A.nbz - 1) ? Tsh[base + 512LL * nbxy + lane] : kInf;
-        // x / y face halos: 4 faces x 64 cells (u = in-face coordinate, w = z)
-        for (int hid = lane; hid < 256; hid += 64) {
-            const int f = hid >> 6, k = hid & 63, u = k & 7, w = k >> 3;
-            int ti;
-            float hv = kInf;
-            switch (f) {
-            case 0: ti = tix(0, u + 1, w + 1);
-                    if (bx > 0)
-                        hv = A.xf ? A.xf[((long long)shot * A.nblk + b - 1) * 128 + 64 + k]
-                                  : Tsh[base - 512 + 7 + 8 * u + 64 * w];
-                    break;
-            case 1: ti = tix(9, u + 1, w + 1);
-                    if (bx < A.nbx - 1)
-                        hv = A.xf ? A.xf[((long long)shot * A.nblk + b + 1) * 128 + k]
-                                  : Tsh[base + 512 + 8 * u + 64 * w];
-                    break;
-            case 2: ti = tix(u + 1, 0, w + 1);
-                    if (by > 0) hv = Tsh[base - 512LL * A.nbx + u + 56 + 64 * w]; break;
-            default: ti = tix(u + 1, 9, w + 1);
-                    if (by < A.nby - 1) hv = Tsh[base + 512LL * A.nbx + u + 64 * w]; break;
-            }
-            tile[ti] = hv;
-        }
+        vz[0] = (okm & 32u) ? zlo : kInf;
+        vz[9] = (okm & 64u) ? zhi : kInf;
+        tile[tix(0, u + 1, w + 1)] = (okm & 2u) ? hv[0] : kInf;
+        tile[tix(9, u + 1, w + 1)] = (okm & 4u) ? hv[1] : kInf;
+        tile[tix(u + 1, 0, w + 1)] = (okm & 8u) ? hv[2] : kInf;
+        tile[tix(u + 1, 9, w + 1)] = (okm & 16u) ? hv[3] : kInf;
         int it = 0, zflip = 0;
         if constexpr (HYP) {
             tile[tcol] = vz[0];
@@ -756,13 +805,36 @@ __global__ void fill_u32(unsigned *__restrict__ p, long long n, unsigned v) {
         p[i] = v;
 }
 
+// After a lazy 8^3 solve: the (shot, block) items never claimed (flag bit 2
+// clear: never scheduled, never visited) get their cold fill, 64 items per
+// wave and pass.
+__global__ void fill_unwritten8(const unsigned *__restrict__ flags, float *__restrict__ T,
+                                long long nitems, long long nblk, long long vol) {
+    const int lane = threadIdx.x & 63;
+    const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long W = ((long long)gridDim.x * blockDim.x) >> 6;
+    const float4 inf4 = make_float4(kInf, kInf, kInf, kInf);
+    for (long long i0 = w0 * 64; i0 < nitems; i0 += W * 64) {
+        const long long i = i0 + lane;
+        unsigned long long m = __ballot(i < nitems && !(flags[i] & 4u));
+        while (m) {
+            const long long it = i0 + __builtin_ctzll(m);
+            m &= m - 1ull;
+            const long long shot = it / nblk, b = it - shot * nblk;
+            float4 *p = reinterpret_cast<float4 *>(T + shot * vol + b * 512);
+            p[lane] = inf4;
+            p[lane + 64] = inf4;
+        }
+    }
+}
+
 // apply_source_time (block_FIM.cpp:295-369): the source's grid node gets
 // S*distance, its block is the only active one.
 __global__ void seed_sources(const float *__restrict__ src, int nshot, const float *__restrict__ S,
                              float *T, unsigned *flags, unsigned *list0, unsigned *cnt,
                              float dx, float dy, float dz, int B, int nbx, int nby,
                              long long vol, long long nblk, const float *keys, unsigned *kmin,
-                             unsigned *tk, float *xf) {
+                             unsigned *tk, float *xf, int lazy) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) {  // the sources form segment 0 of the first list
         for (int k = 0; k < 3 * kSeg; k++) cnt[k * kCntStride] = 0;
@@ -775,14 +847,23 @@ __global__ void seed_sources(const float *__restrict__ src, int nshot, const flo
     const long long a = blk_addr(ix, iy, iz, B, nbx, nby);
     const float ex = (float)(ix * dx) - sx, ey = (float)(iy * dy) - sy, ez = (float)(iz * dz) - sz;
     const float t0 = S[a] * sqrtf(ex * ex + ey * ey + ez * ez);
-    T[(long long)j * vol + a] = t0;
     const unsigned b = (unsigned)(a / (B * B * B));
+    if (lazy) {  // the source block's cold fill (8^3 blocks; the others are claimed)
+        const float4 inf4 = make_float4(kInf, kInf, kInf, kInf);
+        float4 *p = reinterpret_cast<float4 *>(T + (long long)j * vol + (long long)b * 512);
+        for (int k = 0; k < 128; k++) p[k] = inf4;
+        if (xf) {
+            float4 *q = reinterpret_cast<float4 *>(xf + ((long long)j * nblk + b) * 128);
+            for (int k = 0; k < 32; k++) q[k] = inf4;
+        }
+    }
+    T[(long long)j * vol + a] = t0;
     if (xf && B == 8) {  // the source cell's x-face copy
         const int tx = ix & 7, ty = iy & 7, tz = iz & 7;
         if (tx == 0 || tx == 7)
             xf[((long long)j * nblk + b) * 128 + (tx == 7 ? 64 : 0) + ty + 8 * tz] = t0;
     }
-    flags[(long long)j * nblk + b] = 1u;
+    flags[(long long)j * nblk + b] = lazy ? 1u | 4u | (1u << 3) : 1u;
     list0[j] = ((unsigned)j << kShotShift) | b;
     if (tk) {  // the source block opens the first list with key 0
         tk[(long long)j * nblk + b] = 0u;
@@ -1260,7 +1341,10 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
                            kInfBits);
     }
     float *xf = B == 8 ? bf.xf : nullptr;
-    if (!warm_loaded) {
+    // cold 8^3 starts skip the volume fills (stamped blocks, see FimArgs);
+    // graph replays, whose launches cannot carry a launch index, fill
+    const bool lazy = B == 8 && !warm_loaded && !(o.use_graph && bf.graphs);
+    if (!warm_loaded && !lazy) {
         const long long n4 = (long long)nshot * g.vol / 4;
         hipLaunchKernelGGL(fill_inf, dim3(stride_grid(n4, tpb)), dim3(tpb), 0, ctx->stream,
                            reinterpret_cast<float4 *>(bf.T), n4);
@@ -1273,7 +1357,7 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     ST_HIP(hipMemsetAsync(bf.flags, 0, sizeof(unsigned) * g.nblk * nshot, ctx->stream));
     hipLaunchKernelGGL(seed_sources, dim3((nshot + 63) / 64), dim3(64), 0, ctx->stream, dsrc,
                        nshot, bf.S, bf.T, bf.flags, bf.list0, cnt, ctx->dx, ctx->dy, ctx->dz, B,
-                       g.nbx, g.nby, g.vol, g.nblk, keys, bf.kmin, tk, xf);
+                       g.nbx, g.nby, g.vol, g.nblk, keys, bf.kmin, tk, xf, lazy ? 1 : 0);
     ST_HIP(hipGetLastError());
 
     FimArgs A;
@@ -1298,6 +1382,8 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
     A.src = dsrc;
     A.tk = tk;
     A.xf = xf;
+    A.lazy = lazy ? 1 : 0;
+    A.epoch = 0;
 
     // no more waves than an eighth of the batch's blocks (at least 1,024): a
     // small batch, such as the cold re-solve of rejected warm starts, keeps
@@ -1353,7 +1439,7 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
         mix((long long)__builtin_bit_cast(unsigned, A.band)); mix((long long)__builtin_bit_cast(unsigned, A.eps));
         mix((long long)__builtin_bit_cast(unsigned, A.h));
         mix((long long)(uintptr_t)A.src); mix(hyp ? 1 : 0); mix((long long)(uintptr_t)A.tk);
-        mix((long long)(uintptr_t)A.xf);
+        mix((long long)(uintptr_t)A.xf); mix(A.lazy);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) ST_HIP(hipGraphExecDestroy(ctx->graph_exec));
             if (ctx->graph) ST_HIP(hipGraphDestroy(ctx->graph));
@@ -1377,6 +1463,7 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
         } else {
             for (int k = 0; k < Kg; k++, r++) {
                 A.round = (int)(r % 6);
+                A.epoch = (int)r;
                 launch_round(A);
             }
         }
@@ -1394,6 +1481,12 @@ void fim_batch_on(st_ctx *ctx, const FimBufs &bf, const float *dsrc, int nshot,
         ST_REQUIRE(r < max_rounds, ST_ERR_HIP, "FIM did not terminate within the round cap");
     }
     ST_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
+    if (lazy) {
+        const long long n = (long long)nshot * g.nblk;
+        hipLaunchKernelGGL(fill_unwritten8, dim3(stride_grid(n, 256)), dim3(256), 0, ctx->stream,
+                           bf.flags, bf.T, n, g.nblk, g.vol);
+        ST_HIP(hipGetLastError());
+    }
     if (stats) {
         hipLaunchKernelGGL(sum_wave_stats, dim3(1), dim3(1024), 0, ctx->stream, bf.wstats, nwaves,
                            dstats);

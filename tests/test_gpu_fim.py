@@ -182,6 +182,35 @@ def test_graph_replay_matches_eager(ctx):
     assert np.abs(pg1[0] - pe[0]).max() < 1e-5
 
 
+def test_unreached_blocks_hold_the_sentinel(ctx):
+    """Cold 8^3 solves fill a block when it is first scheduled, and blocks
+    never scheduled after the solve.  A slowness of 2,000 s/m keeps travel
+    times below the 1e6 s "not reached" sentinel only within 500 m of the
+    corner source, so most blocks are never scheduled.  The buffers first
+    hold another model's solve (finite times everywhere): the far blocks
+    must read exactly 1e6, and the whole volume must equal the solve that
+    fills every block before its first round (graph replay)."""
+    vel, h = _case()
+    srcs = np.array([[0.0, 0.0, 0.0], [12.5, 20.0, 5.0]], np.float32)
+    ctx.set_model(1.0 / vel, h)
+    _, Tn, _ = ctx.fim_solve(srcs, None, return_T=True, batch=2)
+    assert Tn.max() < 1e5
+    slow = np.full(vel.shape, 2000.0, np.float32)
+    ctx.set_model(slow, h)
+    _, Tl, st = ctx.fim_solve(srcs, None, return_T=True, batch=2)
+    _, Tg, _ = ctx.fim_solve(srcs, None, return_T=True, batch=2, use_graph=1)
+    ny, nx, nz = vel.shape
+    Y, X, Z = np.meshgrid(np.arange(ny) * h, np.arange(nx) * h, np.arange(nz) * h, indexing="ij")
+    for i, s in enumerate(srcs):
+        far = 2000.0 * np.sqrt((X - s[0]) ** 2 + (Y - s[1]) ** 2 + (Z - s[2]) ** 2) > 1.2e6
+        print(f"shot {i}: {far.mean():.0%} of cells beyond the sentinel, stats {st}")
+        assert far.mean() > 0.5
+        assert np.all(Tl[i][far] == np.float32(1e6))
+        assert Tl[i].max() == np.float32(1e6)
+        assert np.array_equal(Tl[i] == np.float32(1e6), Tg[i] == np.float32(1e6))
+        assert np.allclose(Tl[i], Tg[i], rtol=1e-6, atol=0.0)
+
+
 def test_warm_values_judge_and_fallback(ctx):
     """Seeded (1+delta) values: an upper-bound prediction passes the judge;
     an under-estimate (half the true times) is not a valid start for the
