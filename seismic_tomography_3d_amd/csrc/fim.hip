@@ -689,23 +689,24 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
 
         // face activation as in the 4^3 kernel: a changed face cell must be
         // below the neighbour's adjacent cell (tile halo / z halo registers)
-        bool anydec = false, fxm = false, fxp = false, fym = false, fyp = false;
+        bool anydec = false, axl = false, ayl = false;
         // arrival-ordering keys: smallest activating value on the x faces
         // (tx = 0 / 7 lanes), the y faces (ty = 0 / 7) and among the changed
         // cells of the column
         float kx = kKeyInf, ky = kKeyInf, kc = kKeyInf;
+        // a face lane compares with the one halo beyond its own face
+        const bool xface = tx == 0 || tx == 7, yface = ty == 0 || ty == 7;
+        const int ox = tx == 0 ? -kTX : kTX, oy = ty == 0 ? -kTY : kTY;
         #pragma unroll
         for (int k = 0; k < 8; k++) {
             const float dl = v0[k] - v[k];
             anydec |= dl > 0.0f;
             const bool c = dl > A.eps;
             const int ti = tcol + kTZ * (k + 1);
-            const bool ax = c && ((tx == 0 && v[k] < tile[ti - kTX]) || (tx == 7 && v[k] < tile[ti + kTX]));
-            const bool ay = c && ((ty == 0 && v[k] < tile[ti - kTY]) || (ty == 7 && v[k] < tile[ti + kTY]));
-            fxm |= ax && tx == 0;
-            fxp |= ax && tx == 7;
-            fym |= ay && ty == 0;
-            fyp |= ay && ty == 7;
+            const bool ax = c && xface && v[k] < tile[ti + ox];
+            const bool ay = c && yface && v[k] < tile[ti + oy];
+            axl |= ax;
+            ayl |= ay;
             if (A.tk) {
                 kx = ax ? fminf(kx, v[k]) : kx;
                 ky = ay ? fminf(ky, v[k]) : ky;
@@ -726,8 +727,11 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
                     make_float2(tile[ti], tile[ti + kTY]);
             }
         }
-        const bool xm = __ballot(fxm) != 0ull, xp = __ballot(fxp) != 0ull;
-        const bool ym = __ballot(fym) != 0ull, yp = __ballot(fyp) != 0ull;
+        // faces by lane: tx = 0 are lanes 8 ty, tx = 7 lanes 8 ty + 7, ty = 0
+        // lanes 0..7, ty = 7 lanes 56..63
+        const unsigned long long bx_ = __ballot(axl), by_ = __ballot(ayl);
+        const bool xm = (bx_ & 0x0101010101010101ull) != 0ull, xp = (bx_ & 0x8080808080808080ull) != 0ull;
+        const bool ym = (by_ & 0x00000000000000FFull) != 0ull, yp = (by_ & 0xFF00000000000000ull) != 0ull;
         const bool zm = __ballot(fzm) != 0ull, zp = __ballot(fzp) != 0ull;
         unsigned target = 0xFFFFFFFFu;
         switch (lane) {
