@@ -582,10 +582,12 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         }
         vz[0] = (okm & 32u) ? zlo : kInf;
         vz[9] = (okm & 64u) ? zhi : kInf;
-        tile[tix(0, u + 1, w + 1)] = (okm & 2u) ? hv[0] : kInf;
-        tile[tix(9, u + 1, w + 1)] = (okm & 4u) ? hv[1] : kInf;
-        tile[tix(u + 1, 0, w + 1)] = (okm & 8u) ? hv[2] : kInf;
-        tile[tix(u + 1, 9, w + 1)] = (okm & 16u) ? hv[3] : kInf;
+        #pragma unroll
+        for (int f = 0; f < 4; f++) hv[f] = (okm & (2u << f)) ? hv[f] : kInf;
+        tile[tix(0, u + 1, w + 1)] = hv[0];
+        tile[tix(9, u + 1, w + 1)] = hv[1];
+        tile[tix(u + 1, 0, w + 1)] = hv[2];
+        tile[tix(u + 1, 9, w + 1)] = hv[3];
         int it = 0, zflip = 0;
         if constexpr (HYP) {
             tile[tcol] = vz[0];
@@ -596,12 +598,10 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             // octant: along each axis the front enters through the face
             // holding the earlier arrivals (diving waves reach shallow blocks
             // from below); on a tie, away from the source
+            // (the face halos from the registers that staged them)
             auto wmin = [](float v) { return wave_min_all(v); };
-            const int fu = lane & 7, fw = lane >> 3;
-            const float mxm = wmin(tile[tix(0, fu + 1, fw + 1)]);
-            const float mxp = wmin(tile[tix(9, fu + 1, fw + 1)]);
-            const float mym = wmin(tile[tix(fu + 1, 0, fw + 1)]);
-            const float myp = wmin(tile[tix(fu + 1, 9, fw + 1)]);
+            const float mxm = wmin(hv[0]), mxp = wmin(hv[1]);
+            const float mym = wmin(hv[2]), myp = wmin(hv[3]);
             const float mzm = wmin(vz[0]), mzp = wmin(vz[9]);
             const float *sp = A.src + 3 * shot;
             const bool fx = mxm != mxp ? mxm < mxp : (bx * 8 + 3.5f) * A.h >= sp[0];
@@ -748,19 +748,27 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
         if (A.tk) {
             // x faces: min over ty of the tx = 0 / 7 lanes (lanes tx + 8 ty;
             // row_ror 8 pairs ty with ty ^ 1 inside a row, readlane the rest);
-            // y faces: min over tx of the ty = 0 / 7 rows of 8 lanes
-            const float mx = fminf(kx, dpp_mov<0x128>(kx));  // row_ror:8
-            const float my = min8(ky);
-            const float mzm = wave_min_all(fzm ? v[0] : kKeyInf);
-            const float mzp = wave_min_all(fzp ? v[7] : kKeyInf);
-            const float mc = wave_min_all(kc);
-            const float k1 = fminf(fminf(lane_val(mx, 0), lane_val(mx, 16)),
-                                   fminf(lane_val(mx, 32), lane_val(mx, 48)));
-            const float k2 = fminf(fminf(lane_val(mx, 7), lane_val(mx, 23)),
-                                   fminf(lane_val(mx, 39), lane_val(mx, 55)));
-            const float k3 = lane_val(my, 0), k4 = lane_val(my, 56);
-            key = lane == 0 ? mc : lane == 1 ? k1 : lane == 2 ? k2 : lane == 3 ? k3
-                : lane == 4 ? k4 : lane == 5 ? mzm : mzp;
+            // y faces: min over tx of the ty = 0 / 7 rows of 8 lanes.  Only
+            // for the faces that schedule a neighbour (the same in every lane)
+            float k0 = kKeyInf, k1 = kKeyInf, k2 = kKeyInf, k3 = kKeyInf, k4 = kKeyInf,
+                  k5 = kKeyInf, k6 = kKeyInf;
+            if (xm || xp) {
+                const float mx = fminf(kx, dpp_mov<0x128>(kx));  // row_ror:8
+                k1 = fminf(fminf(lane_val(mx, 0), lane_val(mx, 16)),
+                           fminf(lane_val(mx, 32), lane_val(mx, 48)));
+                k2 = fminf(fminf(lane_val(mx, 7), lane_val(mx, 23)),
+                           fminf(lane_val(mx, 39), lane_val(mx, 55)));
+            }
+            if (ym || yp) {
+                const float my = min8(ky);
+                k3 = lane_val(my, 0);
+                k4 = lane_val(my, 56);
+            }
+            if (zm) k5 = wave_min_all(fzm ? v[0] : kKeyInf);
+            if (zp) k6 = wave_min_all(fzp ? v[7] : kKeyInf);
+            if (more) k0 = wave_min_all(kc);
+            key = lane == 0 ? k0 : lane == 1 ? k1 : lane == 2 ? k2 : lane == 3 ? k3
+                : lane == 4 ? k4 : lane == 5 ? k5 : k6;
         }
         schedule(A, shot, fl, target, bit_nxt, nxt, nxt_cnt, lane, key);
         my_visits += 1;
