@@ -45,7 +45,7 @@ typedef struct {
     float delta;        /* warm start inflation (1+delta) (0 = 0.02)           */
     int judge;          /* 1: verify shots started from seeded values
                            (warm_values = 1), re-solve cold on failure         */
-    int grid;           /* waves per round launch (0 = 32 per CU)              */
+    int grid;           /* waves per round launch (0 = 192 per CU)             */
     int rounds_per_sync;/* rounds enqueued between host checks (0 = auto)      */
     int use_graph;      /* 1: replay rounds from a captured hipGraph           */
     float band;         /* warm starts: visit a block only once its predicted
