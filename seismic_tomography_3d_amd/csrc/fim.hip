@@ -184,21 +184,37 @@ __device__ __forceinline__ float dpp_mov(float v) {
 __device__ __forceinline__ float lane_val(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+// Minima over lanes of values that are never negative (travel times and
+// +inf): their bit patterns order as unsigned integers, and an unsigned min
+// whose DPP source falls back to its identity 0xFFFFFFFF compiles to one
+// v_min_u32_dpp per step (a float min takes a copy, a DPP move and the min).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ unsigned dpp_umin(unsigned v) {
+    return min(v, (unsigned)__builtin_amdgcn_update_dpp((int)0xFFFFFFFFu, (int)v, CTRL, ROWS, 0xF,
+                                                         false));
+}
 // minimum over each group of 8 lanes (quad swaps 1 and 2, then the half-row
 // mirror pairs the two quads)
-__device__ __forceinline__ float min8(float v) {
-    v = fminf(v, dpp_mov<0xB1>(v));    // quad_perm [1, 0, 3, 2]
-    v = fminf(v, dpp_mov<0x4E>(v));    // quad_perm [2, 3, 0, 1]
-    return fminf(v, dpp_mov<0x141>(v));  // row_half_mirror
+__device__ __forceinline__ float min8(float f) {
+    unsigned v = __float_as_uint(f);
+    v = dpp_umin<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v = dpp_umin<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    v = dpp_umin<0x141>(v);   // row_half_mirror
+    return __uint_as_float(v);
 }
-// minimum over the whole wave, the same in every lane
-__device__ __forceinline__ float wave_min_all(float v) {
-    v = min8(v);
-    v = fminf(v, dpp_mov<0x140>(v));   // row_mirror: each row's 16 lanes
-    return fminf(fminf(lane_val(v, 0), lane_val(v, 16)), fminf(lane_val(v, 32), lane_val(v, 48)));
+// minimum over the whole wave (values never negative), the same in every
+// lane: rows by DPP, then row_bcast:15 combines rows 1 and 3 with the last
+// lane of the row before and row_bcast:31 rows 2 and 3 with lane 31, so lane
+// 63 ends with the whole wave's minimum (the other rows keep theirs)
+__device__ __forceinline__ float wave_min_all(float f) {
+    unsigned v = __float_as_uint(min8(f));
+    v = dpp_umin<0x140>(v);         // row_mirror: each row's 16 lanes
+    v = dpp_umin<0x142, 0xA>(v);    // row_bcast:15 into rows 1 and 3
+    v = dpp_umin<0x143, 0xC>(v);    // row_bcast:31 into rows 2 and 3
+    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)v, 63));
 }
 
-// maximum over the whole wave, the same in every lane
+// maximum over the whole wave, the same in every lane (any sign)
 __device__ __forceinline__ float wave_max_all(float v) {
     v = fmaxf(v, dpp_mov<0xB1>(v));
     v = fmaxf(v, dpp_mov<0x4E>(v));
@@ -753,7 +769,7 @@ __global__ __launch_bounds__(64 * kWaves, 6) void fim_round8(FimArgs A) {
             float k0 = kKeyInf, k1 = kKeyInf, k2 = kKeyInf, k3 = kKeyInf, k4 = kKeyInf,
                   k5 = kKeyInf, k6 = kKeyInf;
             if (xm || xp) {
-                const float mx = fminf(kx, dpp_mov<0x128>(kx));  // row_ror:8
+                const float mx = __uint_as_float(dpp_umin<0x128>(__float_as_uint(kx)));  // row_ror:8
                 k1 = fminf(fminf(lane_val(mx, 0), lane_val(mx, 16)),
                            fminf(lane_val(mx, 32), lane_val(mx, 48)));
                 k2 = fminf(fminf(lane_val(mx, 7), lane_val(mx, 23)),
