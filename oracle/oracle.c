@@ -116,10 +116,11 @@ void or_source_seed(const float *S, int nxx, int nyy, int nzz,
 }
 
 /* ------------------------------------------------------------------------ */
-/* FP64 fixed point of the same discretisation (the judge of the 1e-4 s     */
-/* gate).  Domain = padded extent, slowness replicated, missing neighbours  */
-/* at the domain edge ignored (equivalent to the reference's "halo = own    */
-/* value", cuda_kernel.cu:294-297, because the update is min(old, G)).      */
+/* FP64 fixed point of the same discretisation (what the 1e-4 s gate is     */
+/* measured against).  Domain = padded extent, slowness replicated, missing */
+/* neighbours at the domain edge ignored (equivalent to the reference's     */
+/* "halo = own value", cuda_kernel.cu:294-297, because the update is        */
+/* min(old, G)).                                                            */
 /* Pinned: exact n*h*s on axes of a homogeneous medium, exact sum of h*s    */
 /* down a vertical line of a layered medium, order independence (forward vs */
 /* reverse sweep start), agreement with or_block_fim_ref.                   */
